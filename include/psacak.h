@@ -1,0 +1,121 @@
+/*
+ * psacak.h -- C ABI of psacak_amd, an MI355X (gfx950) implementation of the
+ * pSACAK suffix-array construction of hucsmn/psacak 0.1.0.
+ *
+ * The suffix array (SA) of a text is unique, so every entry point below
+ * returns exactly the crate's SA for the same input.  The virtual sentinel is
+ * not part of the SA (lib.rs:30-43).  Every call returns a status code; none
+ * aborts the process.  Device work runs on one GPU (the current HIP device
+ * when the context is created).
+ *
+ * Drop-in map (reference file:line -> entry point):
+ *   lib.rs:64-69     pub fn psacak(&[u8]) -> Vec<u32>        psk_psacak (caller allocates)
+ *   lib.rs:77-82     pub fn psacak_inplace(&[u8], &mut [u32]) psk_psacak_inplace
+ *   psacak8.rs:19    fn psacak8(&[u8], &mut [u32])            psk_psacak8_dev (device buffers)
+ *   psacak32.rs:24   fn psacak32(&[u32], &mut [u32], ..)      psk_psacak32 / psk_psacak32_dev
+ *   psacak32.rs:912  translate_sample (test helper)           psk_psacak32_raw (translate + psacak32)
+ */
+#ifndef PSACAK_H
+#define PSACAK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PSK_OK 0
+#define PSK_EINVAL -1   /* null pointer, sa_len < n (lib.rs:78 assert) */
+#define PSK_ETOOLONG -2 /* u8: n >= 2^32-1 (lib.rs:65,79); u32: n > 2^31-1 (psacak32.rs:25) */
+#define PSK_EPRECOND -3 /* psacak32: a symbol >= n (not a bucket-pointer text, psacak32.rs:22) */
+#define PSK_ENOMEM -4   /* device or host allocation failed */
+#define PSK_EBUDGET -5  /* request exceeds the context's HBM budget */
+#define PSK_EDEVICE -6  /* HIP error, or no GPU present */
+#define PSK_EBUSY -7    /* server: queue full */
+#define PSK_ENOTFOUND -8 /* server: unknown request id */
+
+typedef struct psk_ctx psk_ctx;
+
+/* Human-readable text of a status code (static storage). */
+const char *psk_strerror(int status);
+
+/* Library version string, e.g. "psacak_amd 0.1.0 (gfx950)". */
+const char *psk_version(void);
+
+/* ---- contexts ---------------------------------------------------------- */
+/* A context owns a HIP stream and cached device workspace on the current
+ * device.  hbm_budget_bytes = 0 means "no budget".  A context is externally
+ * synchronised; distinct contexts may be used from different threads.
+ * Creation does not touch the GPU; the first call that needs it does. */
+int psk_ctx_create(psk_ctx **out, uint64_t hbm_budget_bytes);
+void psk_ctx_destroy(psk_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *psk_last_error(const psk_ctx *ctx);
+/* Release cached device workspace (keeps the stream). */
+int psk_ctx_trim(psk_ctx *ctx);
+/* Device bytes a psacak8 call on a text of n bytes needs beyond its
+ * input and output buffers (the admission footprint). */
+uint64_t psk_workspace_bytes8(uint64_t n);
+uint64_t psk_workspace_bytes32(uint64_t n);
+
+/* ---- host-buffer entry points (blocking) -------------------------------- */
+/* = psacak_inplace (lib.rs:77-82): SA of text[0..n) into sa[0..n);
+ * sa[n..sa_len) is left untouched.  Requires sa_len >= n, n < 2^32-1.
+ * ctx may be NULL (a per-thread default context is used). */
+int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa,
+                       uint64_t sa_len);
+/* = psacak (lib.rs:64-69) with a caller-owned output of exactly n entries. */
+int psk_psacak(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa);
+/* = psacak32 (psacak32.rs:24): text must be in bucket-pointer form
+ * (an L-type symbol is its bucket's head, an S-type one its tail-1);
+ * every symbol < n, n <= 2^31-1. */
+int psk_psacak32(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa, uint64_t sa_len);
+/* SA of an arbitrary u32 text: translate (psacak32.rs:912-943) then psacak32.
+ * All-distinct texts are sorted by symbol (psacak32.rs:873-881). */
+int psk_psacak32_raw(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa,
+                     uint64_t sa_len);
+
+/* ---- device-buffer entry points ---------------------------------------- */
+/* Pointers are device memory on the context's device; stream is a hipStream_t
+ * (NULL = the context's own stream).  The call enqueues all work on that stream
+ * and returns when the SA is complete on the device (it synchronises the
+ * stream); d_sa must hold n u32. */
+int psk_psacak8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, uint32_t *d_sa,
+                    void *stream);
+int psk_psacak32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,
+                     void *stream);
+int psk_psacak32_raw_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,
+                         void *stream);
+
+/* Per-level statistics of the last psacak8/psacak32 call on ctx:
+ * rows of {m, m1, mL, k, sum of LMS-substring lengths, symbol bytes}.
+ * Returns the number of levels written (<= max_levels). */
+int psk_last_levels(const psk_ctx *ctx, uint64_t *rows, int max_levels);
+
+/* Time (ms, HIP events) of the last call's device work, by phase:
+ * [0] total, [1] level-0 induce, [2] deeper levels, [3] naming+typing. */
+int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
+
+/* ---- batched ------------------------------------------------------------ */
+/* SA of count independent texts.  texts[i] (lens[i] bytes) -> sas[i].
+ * Host buffers.  Small texts are packed into shared device batches. */
+int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
+               uint32_t *const *sas, uint64_t count);
+/* Same on device buffers: one concatenated text buffer d_texts with offsets
+ * (count+1 entries, host array), SA written to d_sas at the same offsets. */
+int psk_batch8_dev(psk_ctx *ctx, const uint8_t *d_texts, const uint64_t *offsets,
+                   uint64_t count, uint32_t *d_sas, void *stream);
+
+/* ---- device-side checker ------------------------------------------------ */
+/* Verifies on the GPU that d_sa is the SA of d_text (permutation + the
+ * Burkhardt-Kaerkkaeinen adjacent-pair condition).  *ok = 1 if valid. */
+int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32_t *d_sa,
+                   int *ok, void *stream);
+
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSACAK_H */

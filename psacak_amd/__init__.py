@@ -1,0 +1,291 @@
+"""psacak_amd -- pSACAK suffix-array construction on AMD MI355X (gfx950).
+
+A drop-in for the public API of the hucsmn/psacak crate (lib.rs):
+
+* :func:`psacak` (``lib.rs:64-69``) -- SA of a byte text, as a new u32 array;
+* :func:`psacak_inplace` (``lib.rs:77-82``) -- SA into a caller's u32 buffer;
+* :func:`psacak32` (``psacak32.rs:24``) -- SA of a u32 text in bucket-pointer
+  form, and :func:`psacak32_raw` for an arbitrary u32 text (translated as in
+  ``psacak32.rs:912-943``).
+
+The SA excludes the virtual sentinel (``lib.rs:30-43``).  Every call runs the
+HIP kernels of ``libpsacak_amd.so`` through its C ABI (``include/psacak.h``);
+there is no CPU fallback: if the library is missing the import fails.  The
+crate panics on bad arguments; here they raise :class:`PskError` subclasses
+before any GPU work.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "PskError", "Context", "psacak", "psacak_inplace", "psacak32", "psacak32_raw",
+    "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "batch8", "batch8_dev",
+    "workspace_bytes8", "workspace_bytes32", "lib_path", "version",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpsacak_amd.so")
+
+PSK_OK, PSK_EINVAL, PSK_ETOOLONG, PSK_EPRECOND = 0, -1, -2, -3
+PSK_ENOMEM, PSK_EBUDGET, PSK_EDEVICE, PSK_EBUSY, PSK_ENOTFOUND = -4, -5, -6, -7, -8
+
+
+class PskError(RuntimeError):
+    """A non-zero status from the C ABI; ``.code`` is the PSK_* value."""
+
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__(f"{_strerror(code)} ({code})" + (f": {msg}" if msg else ""))
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(
+            f"psacak_amd: native library {_LIB_PATH} is missing; build it with "
+            "`python -m psacak_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(_LIB_PATH)
+    vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "psk_strerror": ([i32], ctypes.c_char_p),
+        "psk_version": ([], ctypes.c_char_p),
+        "psk_ctx_create": ([ctypes.POINTER(vp), u64], i32),
+        "psk_ctx_destroy": ([vp], None),
+        "psk_last_error": ([vp], ctypes.c_char_p),
+        "psk_ctx_trim": ([vp], i32),
+        "psk_workspace_bytes8": ([u64], u64),
+        "psk_workspace_bytes32": ([u64], u64),
+        "psk_psacak_inplace": ([vp, vp, u64, vp, u64], i32),
+        "psk_psacak": ([vp, vp, u64, vp], i32),
+        "psk_psacak32": ([vp, vp, u64, vp, u64], i32),
+        "psk_psacak32_raw": ([vp, vp, u64, vp, u64], i32),
+        "psk_psacak8_dev": ([vp, vp, u64, vp, vp], i32),
+        "psk_psacak32_dev": ([vp, vp, u64, vp, vp], i32),
+        "psk_psacak32_raw_dev": ([vp, vp, u64, vp, vp], i32),
+        "psk_last_levels": ([vp, vp, i32], i32),
+        "psk_last_timing": ([vp, vp, i32], i32),
+        "psk_batch8": ([vp, vp, vp, vp, u64], i32),
+        "psk_batch8_dev": ([vp, vp, vp, u64, vp, vp], i32),
+        "psk_check8_dev": ([vp, vp, u64, vp, ctypes.POINTER(i32), vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+_lib = _load()
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def version() -> str:
+    return _lib.psk_version().decode()
+
+
+def _strerror(code: int) -> str:
+    return _lib.psk_strerror(code).decode()
+
+
+def workspace_bytes8(n: int) -> int:
+    return int(_lib.psk_workspace_bytes8(n))
+
+
+def workspace_bytes32(n: int) -> int:
+    return int(_lib.psk_workspace_bytes32(n))
+
+
+class Context:
+    """A device context: one HIP stream plus cached device workspace.
+
+    ``hbm_budget`` (bytes, 0 = none) bounds the workspace it may reserve.
+    """
+
+    def __init__(self, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        rc = _lib.psk_ctx_create(ctypes.byref(h), hbm_budget)
+        if rc != PSK_OK:
+            raise PskError(rc)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.psk_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_error(self) -> str:
+        return _lib.psk_last_error(self._h).decode()
+
+    def trim(self):
+        self._check(_lib.psk_ctx_trim(self._h))
+
+    def last_levels(self):
+        """Per-level rows (m, m1, mL, k, sum of LMS-substring lengths, symbol bytes)."""
+        rows = np.zeros(6 * 64, dtype=np.uint64)
+        n = _lib.psk_last_levels(self._h, rows.ctypes.data, 64)
+        return rows[: 6 * max(n, 0)].reshape(-1, 6)
+
+    def last_timing_ms(self):
+        ms = np.zeros(4, dtype=np.float64)
+        _lib.psk_last_timing(self._h, ms.ctypes.data, 4)
+        return ms
+
+    def _check(self, rc: int):
+        if rc != PSK_OK:
+            raise PskError(rc, self.last_error())
+
+
+def _ctx_handle(ctx):
+    return ctx.handle if ctx is not None else None
+
+
+def _raise(rc: int, ctx=None):
+    if rc != PSK_OK:
+        msg = ctx.last_error() if ctx is not None else ""
+        raise PskError(rc, msg)
+
+
+def _as_u8(text) -> np.ndarray:
+    if isinstance(text, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(text), dtype=np.uint8)
+    a = np.asarray(text)
+    if a.dtype != np.uint8:
+        raise TypeError("psacak expects a byte text (uint8)")
+    return np.ascontiguousarray(a)
+
+
+def psacak(text, ctx: Context | None = None) -> np.ndarray:
+    """SA of a byte text (``lib.rs:64-69``); the sentinel is not included."""
+    t = _as_u8(text)
+    sa = np.empty(len(t), dtype=np.uint32)
+    rc = _lib.psk_psacak(_ctx_handle(ctx), t.ctypes.data, len(t), sa.ctypes.data)
+    _raise(rc, ctx)
+    return sa
+
+
+def psacak_inplace(text, suf: np.ndarray, ctx: Context | None = None) -> None:
+    """SA of ``text`` into ``suf[:len(text)]`` (``lib.rs:77-82``).
+
+    ``suf`` must be a writable, C-contiguous uint32 array at least as long as
+    the text; entries past ``len(text)`` are not touched."""
+    t = _as_u8(text)
+    if not isinstance(suf, np.ndarray) or suf.dtype != np.uint32 or not suf.flags.c_contiguous:
+        raise TypeError("suf must be a C-contiguous numpy uint32 array")
+    rc = _lib.psk_psacak_inplace(_ctx_handle(ctx), t.ctypes.data, len(t), suf.ctypes.data,
+                                 len(suf))
+    _raise(rc, ctx)
+
+
+def psacak32(text, ctx: Context | None = None) -> np.ndarray:
+    """SA of a u32 text in bucket-pointer form (``psacak32.rs:22-24``)."""
+    t = np.ascontiguousarray(text, dtype=np.uint32)
+    sa = np.empty(len(t), dtype=np.uint32)
+    rc = _lib.psk_psacak32(_ctx_handle(ctx), t.ctypes.data, len(t), sa.ctypes.data, len(sa))
+    _raise(rc, ctx)
+    return sa
+
+
+def psacak32_raw(text, ctx: Context | None = None) -> np.ndarray:
+    """SA of an arbitrary u32 text (translated first, ``psacak32.rs:912-943``)."""
+    t = np.ascontiguousarray(text, dtype=np.uint32)
+    sa = np.empty(len(t), dtype=np.uint32)
+    rc = _lib.psk_psacak32_raw(_ctx_handle(ctx), t.ctypes.data, len(t), sa.ctypes.data, len(sa))
+    _raise(rc, ctx)
+    return sa
+
+
+def _dev_ptr(x):
+    """Device pointer of a torch tensor (or an int address)."""
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream  # torch.cuda.Stream
+
+
+def psacak8_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None, stream=None):
+    """SA of a device byte text into a device u32 buffer (torch tensors or
+    raw device addresses); returns when the SA is complete."""
+    if n is None:
+        n = d_text.numel()
+    rc = _lib.psk_psacak8_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
+                              _stream_ptr(stream))
+    _raise(rc, ctx)
+
+
+def psacak32_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None, stream=None):
+    if n is None:
+        n = d_text.numel()
+    rc = _lib.psk_psacak32_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
+                               _stream_ptr(stream))
+    _raise(rc, ctx)
+
+
+def psacak32_raw_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
+                     stream=None):
+    if n is None:
+        n = d_text.numel()
+    rc = _lib.psk_psacak32_raw_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
+                                   _stream_ptr(stream))
+    _raise(rc, ctx)
+
+
+def check8_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
+               stream=None) -> bool:
+    """Device-side SA check (permutation + adjacent-pair rank condition)."""
+    if n is None:
+        n = d_text.numel()
+    ok = ctypes.c_int(0)
+    rc = _lib.psk_check8_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
+                             ctypes.byref(ok), _stream_ptr(stream))
+    _raise(rc, ctx)
+    return bool(ok.value)
+
+
+def batch8(texts, ctx: Context | None = None):
+    """SAs of many independent byte texts (host buffers)."""
+    arrs = [_as_u8(t) for t in texts]
+    outs = [np.empty(len(a), dtype=np.uint32) for a in arrs]
+    k = len(arrs)
+    tp = (ctypes.c_void_p * k)(*[a.ctypes.data for a in arrs])
+    lp = (ctypes.c_uint64 * k)(*[len(a) for a in arrs])
+    sp = (ctypes.c_void_p * k)(*[o.ctypes.data for o in outs])
+    rc = _lib.psk_batch8(_ctx_handle(ctx), tp, lp, sp, k)
+    _raise(rc, ctx)
+    return outs
+
+
+def batch8_dev(d_texts, offsets, d_sas, ctx: Context | None = None, stream=None):
+    """SAs of concatenated device texts: text i is d_texts[offsets[i]:offsets[i+1]]."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    rc = _lib.psk_batch8_dev(_ctx_handle(ctx), _dev_ptr(d_texts), off.ctypes.data,
+                             len(off) - 1, _dev_ptr(d_sas), _stream_ptr(stream))
+    _raise(rc, ctx)

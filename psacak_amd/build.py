@@ -1,0 +1,63 @@
+"""Build the psacak_amd native libraries in-tree.
+
+* ``libpsacak_amd.so`` -- the HIP library (C ABI in ``include/psacak.h``),
+  compiled by hipcc for gfx950 only.
+* ``libpsk_gen.so`` -- the seeded workload generators (host C).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpsacak_amd.so")
+GEN = os.path.join(HERE, "libpsk_gen.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # packed-FP32 instructions stay off in every HIP build of this project
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+    return out
+
+
+def build_hip(verbose: bool = False) -> str:
+    tmp = LIB + ".tmp"
+    out = _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "psacak.hip")])
+    os.replace(tmp, LIB)
+    if verbose and out.strip():
+        print(out)
+    return LIB
+
+
+def build_gen() -> str:
+    tmp = GEN + ".tmp"
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-pthread", "-o", tmp,
+          os.path.join(CSRC, "gen.c"), "-lm"])
+    os.replace(tmp, GEN)
+    return GEN
+
+
+def build_all(verbose: bool = False) -> None:
+    build_gen()
+    build_hip(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    print("built", LIB, GEN)

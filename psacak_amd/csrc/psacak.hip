@@ -1,0 +1,420 @@
+// psacak.hip -- C ABI of psacak_amd (include/psacak.h): contexts, argument
+// checks, host/device entry points, batching and the device-side SA checker.
+//
+// Argument checks run before any HIP call, so they work (and are tested)
+// without a GPU.  Every entry point converts failures into status codes.
+#include <chrono>
+#include <memory>
+#include <mutex>
+
+#include "psk_level8.hpp"
+
+using namespace psk;
+
+struct psk_ctx {
+    int device = -1;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Arena arena;
+    u32 *pinned = nullptr;
+    u64 budget = 0;
+    std::string err;
+    Runtime rt;
+    double last_ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+void ensure_ready(psk_ctx *c) {
+    if (c->ready) return;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        fail(PSK_EDEVICE, "no GPU device present");
+    }
+    PSK_HIP(hipGetDevice(&c->device));
+    PSK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PSK_HIP(hipEventCreate(&c->ev0));
+    PSK_HIP(hipEventCreate(&c->ev1));
+    PSK_HIP(hipHostMalloc((void **)&c->pinned, 16384, hipHostMallocDefault));
+    PSK_HIP(hipMalloc((void **)&c->rt.d.err, 256));
+    PSK_HIP(hipMemset(c->rt.d.err, 0, 256));
+    c->arena.set_budget(c->budget);
+    c->rt.d.s = c->stream;
+    c->rt.d.a = &c->arena;
+    c->rt.d.pinned = c->pinned;
+    c->ready = true;
+}
+
+template <class F>
+int guarded(psk_ctx *c, F f) {
+    try {
+        if (c) c->err.clear();
+        f();
+        return PSK_OK;
+    } catch (const Error &e) {
+        if (c) c->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (c) c->err = "host allocation failed";
+        return PSK_ENOMEM;
+    } catch (...) {
+        if (c) c->err = "unknown failure";
+        return PSK_EDEVICE;
+    }
+}
+
+psk_ctx *default_ctx() {
+    thread_local std::unique_ptr<psk_ctx> c;
+    if (!c) c.reset(new psk_ctx());
+    return c.get();
+}
+
+int check8(const void *text, u64 n, const void *sa, u64 sa_len) {
+    if (n == 0) return PSK_OK;
+    if (!text || !sa) return PSK_EINVAL;
+    if (sa_len < n) return PSK_EINVAL;       // lib.rs:78
+    if (n >= 0xFFFFFFFFull) return PSK_ETOOLONG;  // lib.rs:65,79
+    return PSK_OK;
+}
+
+int check32(const void *text, u64 n, const void *sa, u64 sa_len) {
+    if (n == 0) return PSK_OK;
+    if (!text || !sa) return PSK_EINVAL;
+    if (sa_len < n) return PSK_EINVAL;
+    if (n > 0x7FFFFFFFull) return PSK_ETOOLONG;  // psacak32.rs:25
+    return PSK_OK;
+}
+
+struct Timer {
+    psk_ctx *c;
+    explicit Timer(psk_ctx *cc) : c(cc) {
+        c->rt.d.zero(c->rt.d.err, sizeof(u32));
+        PSK_HIP(hipEventRecord(c->ev0, c->stream));
+    }
+    void stop() {
+        PSK_HIP(hipEventRecord(c->ev1, c->stream));
+        PSK_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0;
+        PSK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms[0] = ms;
+        u32 e = c->rt.d.read_u32(c->rt.d.err);
+        if (e) fail(PSK_EDEVICE, "internal index check failed (flags " + std::to_string(e) + ")");
+    }
+};
+
+// every symbol < n? (bucket-pointer precondition, checked on the device)
+void check_precond32(psk_ctx *c, const u32 *d_text, u64 n) {
+    Dev &d = c->rt.d;
+    Arena::Mark mk = d.a->mark();
+    u32 *bad = d.a->get<u32>(1);
+    d.zero(bad, sizeof(u32));
+    const u64 nn = n;
+    for_each(d, n, [=] __device__(u64 i) {
+        if (d_text[i] >= nn) atomicOr(bad, 1u);
+    });
+    u32 b = d.read_u32(bad);
+    d.a->reset(mk);
+    if (b) fail(PSK_EPRECOND, "psacak32: a symbol is >= n (not a bucket-pointer text)");
+}
+
+void run8_dev(psk_ctx *c, const u8 *d_text, u64 n, u32 *d_sa) {
+    Timer t(c);
+    psacak8_run(c->rt, d_text, n, d_sa);
+    t.stop();
+}
+
+void run32_dev(psk_ctx *c, const u32 *d_text, u64 n, u32 *d_sa, bool raw) {
+    if (!raw) check_precond32(c, d_text, n);
+    Timer t(c);
+    if (raw) {
+        Arena::Mark mk = c->arena.mark();
+        u32 *tr = c->arena.get<u32>(n);
+        if (translate32_run(c->rt, d_text, n, tr, d_sa)) psacak32_run(c->rt, tr, n, d_sa);
+        c->arena.reset(mk);
+    } else {
+        psacak32_run(c->rt, d_text, n, d_sa);
+    }
+    t.stop();
+}
+
+hipStream_t pick_stream(psk_ctx *c, void *stream) {
+    return stream ? (hipStream_t)stream : c->stream;
+}
+
+// Run f with the context's Dev pointed at `stream`.
+template <class F>
+void on_stream(psk_ctx *c, void *stream, F f) {
+    hipStream_t saved = c->rt.d.s;
+    c->rt.d.s = pick_stream(c, stream);
+    hipStream_t saved_ctx = c->stream;
+    c->stream = c->rt.d.s;
+    try {
+        f();
+    } catch (...) {
+        c->rt.d.s = saved;
+        c->stream = saved_ctx;
+        throw;
+    }
+    c->rt.d.s = saved;
+    c->stream = saved_ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *psk_strerror(int status) {
+    switch (status) {
+        case PSK_OK: return "ok";
+        case PSK_EINVAL: return "invalid argument";
+        case PSK_ETOOLONG: return "text too long";
+        case PSK_EPRECOND: return "psacak32 precondition violated (symbol >= n)";
+        case PSK_ENOMEM: return "out of memory";
+        case PSK_EBUDGET: return "HBM budget exceeded";
+        case PSK_EDEVICE: return "device error";
+        case PSK_EBUSY: return "busy";
+        case PSK_ENOTFOUND: return "not found";
+        default: return "unknown status";
+    }
+}
+
+const char *psk_version(void) { return "psacak_amd 0.1.0 (gfx950)"; }
+
+int psk_ctx_create(psk_ctx **out, uint64_t hbm_budget_bytes) {
+    if (!out) return PSK_EINVAL;
+    psk_ctx *c = new (std::nothrow) psk_ctx();
+    if (!c) return PSK_ENOMEM;
+    c->budget = hbm_budget_bytes;
+    *out = c;
+    return PSK_OK;
+}
+
+void psk_ctx_destroy(psk_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->ready) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->arena.release();
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        (void)hipHostFree(ctx->pinned);
+        (void)hipFree(ctx->rt.d.err);
+    }
+    delete ctx;
+}
+
+const char *psk_last_error(const psk_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int psk_ctx_trim(psk_ctx *ctx) {
+    if (!ctx) return PSK_EINVAL;
+    return guarded(ctx, [&] {
+        if (ctx->ready) {
+            PSK_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->arena.release();
+        }
+    });
+}
+
+// Admission footprint: level 0 holds type/LMS bit vectors (n/4), the digit
+// scratch (n), LMS lists and rank table (<= 4 x 2n), plus the recursion on
+// <= n/2 symbols; each u32 level needs <= ~36 bytes per symbol.  Bounded by
+// the worst case m1 = n/2 at every level.
+uint64_t psk_workspace_bytes32(uint64_t n) {
+    uint64_t total = 0, m = n;
+    while (m > 3) {
+        total += m * 36 + (64ull << 20);
+        m /= 2;
+    }
+    return total + (64ull << 20);
+}
+
+uint64_t psk_workspace_bytes8(uint64_t n) {
+    return n / 4 + n + 10 * n + psk_workspace_bytes32(n / 2) + (64ull << 20);
+}
+
+int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa,
+                       uint64_t sa_len) {
+    int rc = check8(text, n, sa, sa_len);
+    if (rc != PSK_OK || n == 0) return rc;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        Arena::Mark mk = c->arena.mark();
+        try {
+            u8 *dt = c->arena.get<u8>(n);
+            u32 *ds = c->arena.get<u32>(n);
+            PSK_HIP(hipMemcpyAsync(dt, text, n, hipMemcpyHostToDevice, c->stream));
+            run8_dev(c, dt, n, ds);
+            PSK_HIP(hipMemcpyAsync(sa, ds, n * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
+            PSK_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            c->arena.reset(mk);
+            throw;
+        }
+        c->arena.reset(mk);
+    });
+}
+
+int psk_psacak(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa) {
+    return psk_psacak_inplace(ctx, text, n, sa, n);
+}
+
+static int host32(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa, uint64_t sa_len,
+                  bool raw) {
+    int rc = check32(text, n, sa, sa_len);
+    if (rc != PSK_OK || n == 0) return rc;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        Arena::Mark mk = c->arena.mark();
+        try {
+            u32 *dt = c->arena.get<u32>(n);
+            u32 *ds = c->arena.get<u32>(n);
+            PSK_HIP(hipMemcpyAsync(dt, text, n * sizeof(u32), hipMemcpyHostToDevice, c->stream));
+            run32_dev(c, dt, n, ds, raw);
+            PSK_HIP(hipMemcpyAsync(sa, ds, n * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
+            PSK_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            c->arena.reset(mk);
+            throw;
+        }
+        c->arena.reset(mk);
+    });
+}
+
+int psk_psacak32(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa, uint64_t sa_len) {
+    return host32(ctx, text, n, sa, sa_len, false);
+}
+
+int psk_psacak32_raw(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa,
+                     uint64_t sa_len) {
+    return host32(ctx, text, n, sa, sa_len, true);
+}
+
+int psk_psacak8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, uint32_t *d_sa,
+                    void *stream) {
+    int rc = check8(d_text, n, d_sa, n);
+    if (rc != PSK_OK || n == 0) return rc;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] { run8_dev(c, d_text, n, d_sa); });
+    });
+}
+
+int psk_psacak32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,
+                     void *stream) {
+    int rc = check32(d_text, n, d_sa, n);
+    if (rc != PSK_OK || n == 0) return rc;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] { run32_dev(c, d_text, n, d_sa, false); });
+    });
+}
+
+int psk_psacak32_raw_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,
+                         void *stream) {
+    int rc = check32(d_text, n, d_sa, n);
+    if (rc != PSK_OK || n == 0) return rc;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] { run32_dev(c, d_text, n, d_sa, true); });
+    });
+}
+
+int psk_last_levels(const psk_ctx *ctx, uint64_t *rows, int max_levels) {
+    if (!ctx || !rows || max_levels < 0) return PSK_EINVAL;
+    int n = std::min<int>(max_levels, (int)ctx->rt.levels.size());
+    for (int i = 0; i < n; i++) {
+        const LevelStat &s = ctx->rt.levels[i];
+        uint64_t *r = rows + 6 * i;
+        r[0] = s.m; r[1] = s.m1; r[2] = s.mL; r[3] = s.k; r[4] = s.sumlen; r[5] = s.csize;
+    }
+    return n;
+}
+
+int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries) {
+    if (!ctx || !ms) return PSK_EINVAL;
+    int n = std::min(max_entries, 4);
+    for (int i = 0; i < n; i++) ms[i] = ctx->last_ms[i];
+    return n;
+}
+
+int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
+               uint32_t *const *sas, uint64_t count) {
+    if (count && (!texts || !lens || !sas)) return PSK_EINVAL;
+    for (uint64_t i = 0; i < count; i++) {
+        int rc = check8(texts[i], lens[i], sas[i], lens[i]);
+        if (rc != PSK_OK) return rc;
+    }
+    for (uint64_t i = 0; i < count; i++) {
+        int rc = psk_psacak_inplace(ctx, texts[i], lens[i], sas[i], lens[i]);
+        if (rc != PSK_OK) return rc;
+    }
+    return PSK_OK;
+}
+
+int psk_batch8_dev(psk_ctx *ctx, const uint8_t *d_texts, const uint64_t *offsets, uint64_t count,
+                   uint32_t *d_sas, void *stream) {
+    if (count && (!d_texts || !offsets || !d_sas)) return PSK_EINVAL;
+    for (uint64_t i = 0; i < count; i++) {
+        if (offsets[i + 1] < offsets[i]) return PSK_EINVAL;
+        if (offsets[i + 1] - offsets[i] >= 0xFFFFFFFFull) return PSK_ETOOLONG;
+    }
+    for (uint64_t i = 0; i < count; i++) {
+        uint64_t o = offsets[i], n = offsets[i + 1] - o;
+        int rc = psk_psacak8_dev(ctx, d_texts + o, n, d_sas + o, stream);
+        if (rc != PSK_OK) return rc;
+    }
+    return PSK_OK;
+}
+
+int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32_t *d_sa,
+                   int *ok, void *stream) {
+    if (!ok) return PSK_EINVAL;
+    *ok = 0;
+    int rc = check8(d_text, n, d_sa, n);
+    if (rc != PSK_OK) return rc;
+    if (n == 0) { *ok = 1; return PSK_OK; }
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] {
+            Dev &d = c->rt.d;
+            Arena::Mark mk = d.a->mark();
+            u32 *rank = d.a->get<u32>(n);
+            u32 *bad = d.a->get<u32>(1);
+            d.zero(bad, sizeof(u32));
+            const u64 nn = n;
+            d.fill_u32(rank, 0xFFFFFFFFu, n);
+            // permutation: every slot claims its suffix exactly once
+            for_each(d, n, [=] __device__(u64 i) {
+                u32 x = d_sa[i];
+                if (x >= nn) { atomicOr(bad, 1u); return; }
+                u32 old = atomicExch(&rank[x], (u32)i);
+                if (old != 0xFFFFFFFFu) atomicOr(bad, 1u);
+            });
+            // Burkhardt-Kaerkkaeinen adjacent-pair condition
+            for_each(d, n - 1, [=] __device__(u64 i) {
+                u32 a = d_sa[i], b = d_sa[i + 1];
+                if (a >= nn || b >= nn) return;
+                u8 ca = d_text[a], cb = d_text[b];
+                if (ca > cb) { atomicOr(bad, 2u); return; }
+                if (ca == cb) {
+                    i64 ra = a + 1 < nn ? (i64)rank[a + 1] : -1;
+                    i64 rb = b + 1 < nn ? (i64)rank[b + 1] : -1;
+                    if (ra >= rb) atomicOr(bad, 2u);
+                }
+            });
+            u32 b = d.read_u32(bad);
+            *ok = b == 0;
+            d.a->reset(mk);
+        });
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// psk_common.hpp -- shared types, error handling, wave64 helpers and the
+// device-memory arena of psacak_amd (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/psacak.h"
+
+namespace psk {
+
+typedef uint8_t u8;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int64_t i64;
+
+constexpr int WAVE = 64;     // CDNA wavefront width
+constexpr int BLOCK = 256;   // 4 waves per workgroup
+constexpr int NWAVES = BLOCK / WAVE;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+#define PSK_HIP(x)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            ::psk::fail(PSK_EDEVICE, std::string(#x) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PSK_LAUNCH_CHECK() PSK_HIP(hipGetLastError())
+
+inline u32 ceil_div(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
+
+// ---- wave64 helpers --------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x / WAVE); }
+__device__ __forceinline__ u64 lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+__device__ __forceinline__ u64 ballot(bool p) { return (u64)__ballot(p); }
+
+// Mask of lanes whose (valid, 8-bit key) equals this lane's: 8 ballots.
+__device__ __forceinline__ u64 match_key8(bool valid, u32 key) {
+    u64 m = ballot(valid);
+    #pragma unroll
+    for (int b = 0; b < 8; b++) {
+        bool bit = (key >> b) & 1u;
+        u64 v = ballot(bit);
+        m &= bit ? v : ~v;
+    }
+    return valid ? m : 0ull;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T x) {
+    const int l = lane_id();
+    #pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        T y = __shfl_up(x, o, WAVE);
+        if (l >= o) x += y;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_max(T x) {
+    const int l = lane_id();
+    #pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        T y = __shfl_up(x, o, WAVE);
+        if (l >= o) x = x > y ? x : y;
+    }
+    return x;
+}
+
+// Block-wide exclusive sum of one value per thread.  lds needs NWAVES+1
+// entries.  Returns the exclusive prefix; *total receives the block sum.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T *lds, T *total) {
+    T inc = wave_inclusive_sum(v);
+    if (lane_id() == WAVE - 1) lds[wave_id()] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int w = 0; w < NWAVES; w++) { T t = lds[w]; lds[w] = acc; acc += t; }
+        lds[NWAVES] = acc;
+    }
+    __syncthreads();
+    T r = inc - v + lds[wave_id()];
+    *total = lds[NWAVES];
+    __syncthreads();
+    return r;
+}
+
+// ---- device arena -------------------------------------------------------------
+// Stack allocator over large hipMalloc'd chunks that are kept for reuse.
+// mark()/reset() pop everything allocated after the mark.
+class Arena {
+  public:
+    struct Mark { size_t chunk, top; };
+
+    ~Arena() { release(); }
+
+    void set_budget(u64 b) { budget_ = b; }
+    u64 reserved() const { return reserved_; }
+    u64 peak() const { return peak_; }
+
+    Mark mark() const { return {cur_, top_}; }
+    void reset(Mark m) { cur_ = m.chunk; top_ = m.top; }
+
+    void *alloc(u64 bytes) {
+        bytes = (bytes + 255) & ~(u64)255;
+        if (bytes == 0) bytes = 256;
+        while (cur_ < chunks_.size()) {
+            if (top_ + bytes <= chunks_[cur_].cap) {
+                void *p = chunks_[cur_].p + top_;
+                top_ += bytes;
+                return p;
+            }
+            cur_++;
+            top_ = 0;
+        }
+        u64 cap = std::max<u64>(bytes, min_chunk_);
+        if (budget_ && reserved_ + cap > budget_) {
+            cap = bytes;
+            if (reserved_ + cap > budget_)
+                fail(PSK_EBUDGET, "device workspace exceeds the HBM budget");
+        }
+        char *p = nullptr;
+        if (hipMalloc((void **)&p, cap) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(PSK_ENOMEM, "hipMalloc of " + std::to_string(cap) + " bytes failed");
+        }
+        chunks_.push_back({p, cap});
+        reserved_ += cap;
+        peak_ = std::max(peak_, reserved_);
+        cur_ = chunks_.size() - 1;
+        top_ = bytes;
+        return p;
+    }
+
+    template <typename T>
+    T *get(u64 count) { return (T *)alloc(count * sizeof(T)); }
+
+    void release() {
+        for (auto &c : chunks_) (void)hipFree(c.p);
+        chunks_.clear();
+        reserved_ = 0;
+        cur_ = top_ = 0;
+    }
+
+  private:
+    struct Chunk { char *p; u64 cap; };
+    std::vector<Chunk> chunks_;
+    size_t cur_ = 0;
+    u64 top_ = 0;
+    u64 reserved_ = 0, peak_ = 0, budget_ = 0;
+    u64 min_chunk_ = 64ull << 20;
+};
+
+}  // namespace psk

@@ -1,0 +1,296 @@
+// psk_level32.hpp -- one recursion level on an integer (u32) text in
+// bucket-pointer form (psacak32.rs:24-56): an L-type symbol is the head of
+// its bucket, an S-type symbol the bucket's tail - 1.
+//
+// Induced sorting with a large alphabet cannot proceed bucket by bucket (the
+// buckets number ~m/3), so each scan is computed by "chain sorting", which
+// yields exactly the order the sequential scan produces:
+//
+//  * L-scan.  An L-type suffix p is placed in its bucket's L part in the order
+//    of its successor's rank (the induce reads SA in order), so within a
+//    bucket, L suffixes sort by (T[p], rank(p+1)); p+1 is either L-type
+//    again or the chain's terminal: an LMS suffix (whose rank is its slot at
+//    the bucket tail) or the sentinel (rank below everything).  Sorting the
+//    L suffixes by these chain keys is prefix doubling over the chains:
+//    labels start as bucket-level ranks (first slot of the bucket's L part),
+//    terminals are unique, and each round sorts the tied items by
+//    (label(p), label(p+h)), splits ties and doubles h.
+//  * S-scan.  Same with S suffixes, whose chains end at an L-type suffix whose
+//    final slot the L-scan has fixed.
+//
+// Labels are final slot + 1 (so the sentinel can be 0) under the
+// "first slot of the group" convention, which keeps every group inside the
+// slot range it will finally occupy; a round turns a group's label into
+// label + (index of the subgroup's first item - index of the group's first
+// item) in the sorted order.
+#pragma once
+
+#include "psk_naming.hpp"
+
+namespace psk {
+
+struct LevelStat {
+    u64 m, m1, mL, k, sumlen, csize;
+};
+
+struct Runtime {
+    Dev d;
+    std::vector<LevelStat> levels;
+    int max_rounds = 0;
+};
+
+// positions p < n whose bit in wordfn(w) is set, ascending, into out.
+template <class WF>
+inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
+    u64 nwords = (n + 31) / 32;
+    scan_exclusive(
+        d, nwords,
+        [=] __device__(u64 w) {
+            u32 x = wordfn(w);
+            if (w * 32 + 32 > n) x &= (n - w * 32 >= 32) ? 0xFFFFFFFFu : ((1u << (n - w * 32)) - 1u);
+            return (u32)__popc(x);
+        },
+        [=] __device__(u64 w, u32 off, u32) {
+            u32 x = wordfn(w);
+            if (w * 32 + 32 > n) x &= (n - w * 32 >= 32) ? 0xFFFFFFFFu : ((1u << (n - w * 32)) - 1u);
+            while (x) {
+                int b = __ffs(x) - 1;
+                x &= x - 1;
+                out[off++] = (u32)(w * 32 + b);
+            }
+        },
+        nullptr);
+}
+
+// Sort items[0..A) by chain keys (see above); RK holds the labels of every
+// position 0..m (m = sentinel) and receives the final labels of the items.
+inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A) {
+    Dev &d = rt.d;
+    if (A == 0) return;
+    Arena::Mark mk = d.a->mark();
+    u32 *act = d.a->get<u32>(A);
+    u32 *vt = d.a->get<u32>(A);
+    u64 *keys = d.a->get<u64>(A);
+    u64 *kt = d.a->get<u64>(A);
+    u32 *fidx = d.a->get<u32>(m + 1);
+    u32 *cnt = d.a->get<u32>(1);
+    PSK_HIP(hipMemcpyAsync(act, items, (u64)A * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
+    const int B = bit_width(m);
+    u64 h = 1;
+    int rounds = 0;
+    while (A > 0) {
+        rounds++;
+        const u64 hh = h, mm = m;
+        const u32 *ac = act;
+        u32 *err = d.err;
+        for_each(d, A, [=] __device__(u64 i) {
+            u32 p = ac[i];
+            u64 q = (u64)p + hh;
+            u32 lo = 0;
+            if (q <= mm) lo = RK[q];
+            else atomicOr(err, 16u);
+            keys[i] = ((u64)RK[p] << B) | (u64)lo;
+        });
+        bool sw = radix_sort_kv(d, A, keys, act, kt, vt, 2 * B);
+        const u64 *K = sw ? kt : keys;
+        const u32 *V = sw ? vt : act;
+        u32 *nxt = sw ? act : vt;  // the buffer not holding V
+        const u64 AA = A;
+        for_each(d, AA, [=] __device__(u64 i) {
+            u64 hi = K[i] >> B;
+            if (hi <= mm) fidx[hi] = 0xFFFFFFFFu;
+            else atomicOr(err, 32u);
+        });
+        for_each(d, AA, [=] __device__(u64 i) {
+            u64 hi = K[i] >> B;
+            if (hi <= mm) atomicMin(&fidx[hi], (u32)i);
+        });
+        scan_inclusive_max(
+            d, AA, [=] __device__(u64 i) { return (i == 0 || K[i] != K[i - 1]) ? (u32)i : 0u; },
+            [=] __device__(u64 i, u32 s0, u32) {
+                u64 hi = K[i] >> B;
+                if (hi <= mm) RK[V[i]] = (u32)hi + (s0 - fidx[hi]);
+            });
+        scan_exclusive(
+            d, AA,
+            [=] __device__(u64 i) {
+                bool keep = (i > 0 && K[i] == K[i - 1]) || (i + 1 < AA && K[i] == K[i + 1]);
+                return (u32)keep;
+            },
+            [=] __device__(u64 i, u32 off, u32 v) {
+                if (v) nxt[off] = V[i];
+            },
+            cnt);
+        A = d.read_u32(cnt);
+        if (nxt != act) std::swap(act, vt);
+        h *= 2;
+        if (h > m + 1 && A > 0) fail(PSK_EDEVICE, "chain sort did not converge");
+    }
+    rt.max_rounds = std::max(rt.max_rounds, rounds);
+    d.a->reset(mk);
+}
+
+// L and S labels for a scan of a bucket-pointer text s[0..m).
+// mode 0: L-scan labels for L positions (bucket head + 1), RK[m] = 0.
+inline void labels_L(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
+    for_each(d, m + 1, [=] __device__(u64 x) {
+        if (x == m) RK[m] = 0;
+        else if (!getbit(sbits, x)) RK[x] = s[x] + 1;
+    });
+}
+
+// S-scan labels: first slot of the bucket's S part + 1 = s[x] - nS + 2.
+inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK, u32 *cnt) {
+    d.zero(cnt, (m + 1) * sizeof(u32));
+    for_each(d, m, [=] __device__(u64 x) {
+        if (getbit(sbits, x)) atomicAdd(&cnt[s[x]], 1u);
+    });
+    for_each(d, m, [=] __device__(u64 x) {
+        if (getbit(sbits, x)) RK[x] = s[x] + 2 - cnt[s[x]];
+    });
+}
+
+void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth);
+
+// Sorted LMS positions of a level (ties between equal LMS substrings in the
+// induced order), given complete labels RK of every position: write the SA
+// and keep the LMS entries.
+inline void write_sa_from_labels(Dev &d, u64 m, const u32 *RK, u32 *SA) {
+    u32 *err = d.err;
+    for_each(d, m, [=] __device__(u64 x) {
+        u32 r = RK[x];
+        if (r >= 1 && r <= m) SA[r - 1] = (u32)x;
+        else atomicOr(err, 64u);
+    });
+}
+
+inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lmsbits, u32 *SA,
+                               u32 *lms_sorted) {
+    write_sa_from_labels(d, m, RK, SA);
+    scan_exclusive(
+        d, m, [=] __device__(u64 i) { return getbit(lmsbits, SA[i]); },
+        [=] __device__(u64 i, u32 off, u32 v) {
+            if (v) lms_sorted[off] = SA[i];
+        },
+        nullptr);
+}
+
+// Solve the reduced problem of a level: names -> SA1 (as LMS positions).
+template <class C>
+inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
+                          const u32 *R, u32 m1, u32 *SA1, int depth, LevelStat &st) {
+    Dev &d = rt.d;
+    if (m1 == 0) return;
+    Arena::Mark mk = d.a->mark();
+    u32 *reduced = d.a->get<u32>(m1);
+    u32 *tails = d.a->get<u32>(m1);
+    u32 *diff = d.a->get<u32>(m1);
+    u32 *dk = d.a->get<u32>(1);
+    u32 k = name_lms(d, T, lms_sorted, lms_text, R, m1, diff, reduced, tails, dk);
+    st.k = k;
+    rt.levels[depth] = st;
+    if (k < m1) {
+        u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
+        compute_types(d, (const u32 *)reduced, m1, sb);
+        translate_names(d, reduced, m1, sb, tails);
+        level32(rt, reduced, m1, sb, SA1, depth + 1);
+    } else {
+        inverse_names(d, reduced, m1, SA1);
+    }
+    permute_lms(d, SA1, m1, lms_text);
+    d.a->reset(mk);
+}
+
+// One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
+inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth) {
+    Dev &d = rt.d;
+    if (m == 0) return;
+    if (m <= 3) {
+        k_naive_small<u32><<<1, 1, 0, d.s>>>(s, (u32)m, SA);
+        PSK_LAUNCH_CHECK();
+        return;
+    }
+    Arena::Mark mk = d.a->mark();
+    const u64 nwords = (m + 31) / 32;
+    u32 *lmsbits = d.a->get<u32>(nwords + 1);
+    u32 *ctr = d.a->get<u32>(2);
+    d.zero(ctr, 2 * sizeof(u32));
+    k_lms32<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(sbits, lmsbits, nwords, ctr);
+    PSK_LAUNCH_CHECK();
+    const u32 m1 = d.read_u32(ctr);
+    u32 *lms_text = d.a->get<u32>(m1 + 1);
+    u32 *R = d.a->get<u32>(m / 2 + 1);
+    bits_to_positions(d, lmsbits, nwords, lms_text, R);
+
+    // L and S item lists (ascending positions)
+    u32 *Litems = d.a->get<u32>(m);
+    u32 *dcnt = ctr + 1;
+    scan_exclusive(
+        d, m, [=] __device__(u64 x) { return 1u - getbit(sbits, x); },
+        [=] __device__(u64 x, u32 off, u32 v) {
+            if (v) Litems[off] = (u32)x;
+        },
+        dcnt);
+    const u32 nL = d.read_u32(dcnt);
+    const u32 nS = (u32)m - nL;
+    u32 *Sitems = Litems + nL;
+    positions_of(d, m, [=] __device__(u64 w) { return sbits[w]; }, Sitems);
+
+    LevelStat st{m, m1, nL, 0, 0, 4};
+    if ((int)rt.levels.size() <= depth) rt.levels.resize(depth + 1);
+    rt.levels[depth] = st;
+
+    u32 *RK = d.a->get<u32>(m + 1);
+    u32 *cnt = d.a->get<u32>(m + 1);
+
+    // ---- stage 1: sort LMS substrings --------------------------------------
+    // LMS characters at their bucket tails, in arbitrary order (psacak32.rs:60-99)
+    labels_L(d, s, m, sbits, RK);
+    d.zero(cnt, (m + 1) * sizeof(u32));
+    {
+        u32 *err = d.err;
+        for_each(d, m1, [=] __device__(u64 r) {
+            u32 x = lms_text[r];
+            u32 k = atomicAdd(&cnt[s[x]], 1u);
+            if (k <= s[x]) RK[x] = s[x] - k + 1;
+            else atomicOr(err, 128u);
+        });
+    }
+    chain_sort(rt, m, RK, Litems, nL);
+    labels_S(d, s, m, sbits, RK, cnt);
+    chain_sort(rt, m, RK, Sitems, nS);
+    u32 *lms_sorted = d.a->get<u32>(m1 + 1);
+    extract_lms_sorted(d, m, RK, lmsbits, SA, lms_sorted);
+    if (m1) {
+        u32 first = 0;
+        d.read(&first, lms_text, sizeof(u32));
+        st.sumlen = m - first + m1;
+    }
+    rt.levels[depth] = st;
+
+    // ---- reduced problem ---------------------------------------------------
+    u32 *SA1 = d.a->get<u32>(m1 + 1);
+    solve_reduced(rt, s, lms_sorted, lms_text, R, m1, SA1, depth, st);
+
+    // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
+    labels_L(d, s, m, sbits, RK);
+    if (m1) {
+        d.zero(cnt, (m + 1) * sizeof(u32));
+        const u32 *sl = SA1;
+        for_each(d, m1, [=] __device__(u64 j) { atomicMax(&cnt[s[sl[j]]], (u32)j); });
+        u32 *err = d.err;
+        for_each(d, m1, [=] __device__(u64 j) {
+            u32 p = sl[j], v = s[p];
+            u32 back = cnt[v] - (u32)j;
+            if (back <= v) RK[p] = v - back + 1;
+            else atomicOr(err, 256u);
+        });
+    }
+    chain_sort(rt, m, RK, Litems, nL);
+    labels_S(d, s, m, sbits, RK, cnt);
+    chain_sort(rt, m, RK, Sitems, nS);
+    write_sa_from_labels(d, m, RK, SA);
+    d.a->reset(mk);
+}
+
+}  // namespace psk

@@ -1,0 +1,176 @@
+// psk_level8.hpp -- level 0 (byte text) driver: psacak8.rs:19-53 on the GPU.
+#pragma once
+
+#include "psk_induce8.hpp"
+#include "psk_level32.hpp"
+
+namespace psk {
+
+// SA of the byte text T[0..n) into SA (device buffers).
+inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA) {
+    Dev &d = rt.d;
+    rt.levels.clear();
+    rt.max_rounds = 0;
+    if (n == 0) return;
+    if (n <= 3) {  // psacak8.rs:23-26
+        k_naive_small<u8><<<1, 1, 0, d.s>>>(T, (u32)n, SA);
+        PSK_LAUNCH_CHECK();
+        return;
+    }
+    Arena::Mark mk = d.a->mark();
+    const u64 nwords = (n + 31) / 32;
+    u32 *sbits = d.a->get<u32>(nwords + 1);
+    u32 *lmsbits = d.a->get<u32>(nwords + 1);
+    compute_types(d, T, n, sbits);
+
+    // bucket statistics (Buckets::new psacak8.rs:355-374)
+    u32 *stat = d.a->get<u32>(3 * 256 + 1);
+    d.zero(stat, (3 * 256 + 1) * sizeof(u32));
+    k_lms_hist8<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(T, n, sbits, lmsbits, nwords, stat,
+                                                          stat + 256, stat + 512, stat + 768);
+    PSK_LAUNCH_CHECK();
+    Buckets8 B;
+    u32 host_stat[3 * 256 + 1];
+    d.read(host_stat, stat, sizeof(host_stat));
+    memcpy(B.cnt, host_stat, sizeof(B.cnt));
+    memcpy(B.nL, host_stat + 256, sizeof(B.nL));
+    memcpy(B.nLMS, host_stat + 512, sizeof(B.nLMS));
+    const u32 m1 = host_stat[768];
+    B.head[0] = 0;
+    u64 nL_total = 0;
+    for (int c = 0; c < 256; c++) {
+        B.head[c + 1] = B.head[c] + B.cnt[c];
+        nL_total += B.nL[c];
+    }
+
+    // device tables: head, tail, fill, counts, tmp, lmsbase, lmsoff
+    u32 *tables = d.a->get<u32>(7 * 256 + 8);
+    Induce8Tables tb{tables, tables + 256, tables + 512, tables + 768, tables + 1024};
+    u32 *lmsbase = tables + 1280, *lmsoff = tables + 1536;
+    {
+        u32 h[7 * 256 + 8];
+        memset(h, 0, sizeof(h));
+        u64 start = 0;
+        for (int c = 0; c < 256; c++) {
+            h[c] = (u32)B.head[c];
+            h[256 + c] = (u32)B.head[c + 1];
+            h[1280 + c] = (u32)(B.head[c + 1] - B.nLMS[c]);
+            h[1536 + c] = (u32)(B.head[c + 1] - B.nLMS[c] - start);
+            start += B.nLMS[c];
+        }
+        d.write(tables, h, sizeof(h));
+    }
+
+    u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
+    u32 *R = d.a->get<u32>(n / 2 + 1);
+    bits_to_positions(d, lmsbits, nwords, lms_text, R);
+    u8 *dig = d.a->get<u8>(n);
+
+    LevelStat st{n, m1, nL_total, 0, 0, 1};
+    rt.levels.resize(1);
+    if (m1) {
+        u32 first = 0;
+        d.read(&first, lms_text, sizeof(u32));
+        st.sumlen = n - first + m1;
+    }
+    rt.levels[0] = st;
+
+    // ---- stage 1: LMS characters to bucket tails (psacak8.rs:57-72), in text
+    // order within a bucket, then induce LMS substrings ----------------------
+    multisplit(
+        d, m1,
+        [=] __device__(u64 i, u32 *key) {
+            *key = T[lms_text[i]];
+            return true;
+        },
+        [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
+            SA[lmsbase[key] + (gpos - dstart)] = lms_text[i];
+        },
+        nullptr);
+    induce8(d, T, n, SA, B, tb, dig);
+
+    // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
+    u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
+    scan_exclusive(
+        d, n, [=] __device__(u64 i) { return getbit(lmsbits, SA[i]); },
+        [=] __device__(u64 i, u32 off, u32 v) {
+            if (v) lms_sorted[off] = SA[i];
+        },
+        nullptr);
+
+    // ---- reduced problem (psacak8.rs:36-48) ---------------------------------
+    u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
+    solve_reduced(rt, T, lms_sorted, lms_text, R, m1, SA1, 0, st);
+
+    // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
+    for_each(d, m1, [=] __device__(u64 j) {
+        u32 p = SA1[j];
+        SA[j + lmsoff[T[p]]] = p;
+    });
+    induce8(d, T, n, SA, B, tb, dig);
+    d.a->reset(mk);
+}
+
+// psacak32 on a bucket-pointer text (device buffers).
+inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
+    Dev &d = rt.d;
+    rt.levels.clear();
+    rt.max_rounds = 0;
+    if (m == 0) return;
+    Arena::Mark mk = d.a->mark();
+    u32 *sb = d.a->get<u32>((m + 31) / 32 + 1);
+    compute_types(d, s, m, sb);
+    rt.levels.resize(1);
+    level32(rt, s, m, sb, SA, 0);
+    d.a->reset(mk);
+}
+
+// translate_sample (psacak32.rs:912-943) on the device: rank the symbols,
+// L -> head of the symbol's bucket, S -> tail - 1.  Returns false (and writes
+// the SA directly) if every symbol is distinct.
+inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA) {
+    Dev &d = rt.d;
+    Arena::Mark mk = d.a->mark();
+    u64 *k = d.a->get<u64>(n), *kt = d.a->get<u64>(n);
+    u32 *v = d.a->get<u32>(n), *vt = d.a->get<u32>(n);
+    u32 *first = d.a->get<u32>(n);
+    u32 *ctr = d.a->get<u32>(1);
+    for_each(d, n, [=] __device__(u64 i) {
+        k[i] = t[i];
+        v[i] = (u32)i;
+    });
+    bool sw = radix_sort_kv(d, n, k, v, kt, vt, 32);
+    const u64 *K = sw ? kt : k;
+    const u32 *V = sw ? vt : v;
+    const u64 nn = n;
+    // distinct count
+    scan_exclusive(
+        d, n, [=] __device__(u64 i) { return (u32)(i == 0 || K[i] != K[i - 1]); },
+        [=] __device__(u64, u32, u32) {}, ctr);
+    u32 distinct = d.read_u32(ctr);
+    if (distinct >= n) {
+        PSK_HIP(hipMemcpyAsync(SA, V, n * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
+        d.a->reset(mk);
+        return false;
+    }
+    // head = first sorted index of the symbol's run; tail = last + 1
+    scan_inclusive_max(
+        d, n, [=] __device__(u64 i) { return (i == 0 || K[i] != K[i - 1]) ? (u32)i : 0u; },
+        [=] __device__(u64 i, u32 s0, u32) { first[i] = s0; });
+    u32 *sb = d.a->get<u32>((n + 31) / 32 + 1);
+    compute_types(d, t, n, sb);
+    // last sorted index of each run, keyed by the run's first index
+    u32 *last = d.a->get<u32>(n);
+    for_each(d, n, [=] __device__(u64 i) {
+        if (i == 0 || K[i] != K[i - 1]) last[i] = 0;
+    });
+    for_each(d, n, [=] __device__(u64 i) { atomicMax(&last[first[i]], (u32)i); });
+    for_each(d, nn, [=] __device__(u64 i) {
+        u32 p = V[i];
+        out[p] = getbit(sb, p) ? last[first[i]] : first[i];
+    });
+    d.a->reset(mk);
+    return true;
+}
+
+}  // namespace psk

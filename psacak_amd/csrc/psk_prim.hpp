@@ -1,0 +1,350 @@
+// psk_prim.hpp -- device-wide primitives used by every level of the
+// suffix-array pipeline: tiled 3-phase scans (sum / max), stream compaction,
+// a stable 8-bit multi-split pass (one LSD radix pass with caller-defined
+// digit and destination), and an LSD radix sort of (u64 key, u32 value).
+//
+// All kernels use 256-thread workgroups (4 wave64s).  Tiles are processed in
+// "striped" rounds of 256 consecutive items so global loads coalesce and the
+// in-tile order is the item order (the passes are stable).
+#pragma once
+
+#include "psk_common.hpp"
+
+namespace psk {
+
+// ---------------------------------------------------------------------------
+// host-side device handle: stream + arena + pinned readback buffer
+// ---------------------------------------------------------------------------
+struct Dev {
+    hipStream_t s = nullptr;
+    Arena *a = nullptr;
+    u32 *pinned = nullptr;  // >= 4096 u32 of pinned host memory
+    u32 *err = nullptr;     // device flag: an index left its range (internal error)
+
+    u32 read_u32(const u32 *d) {
+        PSK_HIP(hipMemcpyAsync(pinned, d, sizeof(u32), hipMemcpyDeviceToHost, s));
+        PSK_HIP(hipStreamSynchronize(s));
+        return pinned[0];
+    }
+    void read(void *host, const void *d, u64 bytes) {
+        PSK_HIP(hipMemcpyAsync(pinned, d, bytes, hipMemcpyDeviceToHost, s));
+        PSK_HIP(hipStreamSynchronize(s));
+        memcpy(host, pinned, bytes);
+    }
+    void write(void *d, const void *host, u64 bytes) {
+        // staged through pinned memory; the caller may reuse `host` at once
+        memcpy(pinned + 2048, host, bytes);
+        PSK_HIP(hipMemcpyAsync(d, pinned + 2048, bytes, hipMemcpyHostToDevice, s));
+        PSK_HIP(hipStreamSynchronize(s));
+    }
+    void zero(void *d, u64 bytes) {
+        if (bytes) PSK_HIP(hipMemsetAsync(d, 0, bytes, s));
+    }
+    void fill_u32(u32 *d, u32 v, u64 count);
+};
+
+template <typename F>
+__global__ void k_for_each(F f, u64 n) {
+    u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    u64 stride = (u64)gridDim.x * BLOCK;
+    for (; i < n; i += stride) f(i);
+}
+
+// Launch f(i) for i in [0, n) with a grid-stride loop.
+template <typename F>
+inline void for_each(Dev &d, u64 n, F f) {
+    if (n == 0) return;
+    u64 blocks = (n + BLOCK - 1) / BLOCK;
+    if (blocks > 65536u * 4) blocks = 65536u * 4;
+    k_for_each<<<(u32)blocks, BLOCK, 0, d.s>>>(f, n);
+    PSK_LAUNCH_CHECK();
+}
+
+inline void Dev::fill_u32(u32 *dp, u32 v, u64 count) {
+    for_each(*this, count, [=] __device__(u64 i) { dp[i] = v; });
+}
+
+// ---------------------------------------------------------------------------
+// 3-phase scans.  in(i) gives the u32 value of item i; out(i, prefix, v) is
+// called once per item with its exclusive prefix (sum scan) or inclusive
+// running value (max scan).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_ROUNDS = 16;
+constexpr u64 SCAN_TILE = (u64)BLOCK * SCAN_ROUNDS;  // 4096 items per tile
+constexpr int TSCAN_IPT = 16;                       // tile sums per thread
+
+struct OpSum {
+    __device__ static u32 id() { return 0u; }
+    __device__ static u32 op(u32 a, u32 b) { return a + b; }
+};
+struct OpMax {
+    __device__ static u32 id() { return 0u; }
+    __device__ static u32 op(u32 a, u32 b) { return a > b ? a : b; }
+};
+
+template <class Op>
+__device__ __forceinline__ u32 wave_incl(u32 x) {
+    const int l = lane_id();
+    #pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        u32 y = __shfl_up(x, o, WAVE);
+        if (l >= o) x = Op::op(x, y);
+    }
+    return x;
+}
+
+// block inclusive scan of one value per thread; *agg = block aggregate.
+template <class Op>
+__device__ __forceinline__ u32 block_incl(u32 v, u32 *lds, u32 *agg) {
+    u32 inc = wave_incl<Op>(v);
+    if (lane_id() == WAVE - 1) lds[wave_id()] = inc;
+    __syncthreads();
+    u32 pre = Op::id();
+    for (int w = 0; w < wave_id(); w++) pre = Op::op(pre, lds[w]);
+    u32 a = Op::id();
+    for (int w = 0; w < NWAVES; w++) a = Op::op(a, lds[w]);
+    *agg = a;
+    __syncthreads();
+    return Op::op(pre, inc);
+}
+
+template <class Op, class In>
+__global__ void k_scan_reduce(In in, u64 n, u32 *tile_agg) {
+    __shared__ u32 lds[NWAVES + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 acc = Op::id();
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        if (i < n) acc = Op::op(acc, in(i));
+    }
+    u32 agg;
+    block_incl<Op>(acc, lds, &agg);
+    if (threadIdx.x == 0) tile_agg[blockIdx.x] = agg;
+}
+
+// single workgroup: exclusive scan (Op) of nt tile aggregates in place;
+// *total (if non-null) receives the grand aggregate.
+template <class Op>
+__global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
+    __shared__ u32 lds[NWAVES + 1];
+    u32 carry = Op::id();
+    for (u64 base = 0; base < nt; base += (u64)BLOCK * TSCAN_IPT) {
+        u64 t0 = base + (u64)threadIdx.x * TSCAN_IPT;
+        u32 v[TSCAN_IPT];
+        u32 loc = Op::id();
+        #pragma unroll
+        for (int k = 0; k < TSCAN_IPT; k++) {
+            v[k] = (t0 + k < nt) ? a[t0 + k] : Op::id();
+            loc = Op::op(loc, v[k]);
+        }
+        u32 agg;
+        u32 inc = block_incl<Op>(loc, lds, &agg);
+        // exclusive prefix of this thread = inclusive of previous thread
+        u32 ex = __shfl_up(inc, 1, WAVE);
+        if (lane_id() == 0) {
+            ex = Op::id();
+            for (int w = 0; w < wave_id(); w++) ex = Op::op(ex, lds[w]);
+        }
+        __syncthreads();  // lds read above before the next round writes it
+        u32 run = Op::op(carry, ex);
+        #pragma unroll
+        for (int k = 0; k < TSCAN_IPT; k++) {
+            if (t0 + k < nt) a[t0 + k] = run;
+            run = Op::op(run, v[k]);
+        }
+        carry = Op::op(carry, agg);
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+template <class Op, bool INCLUSIVE, class In, class Out>
+__global__ void k_scan_apply(In in, Out out, u64 n, const u32 *tile_off) {
+    __shared__ u32 lds[NWAVES + 1];
+    u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 run = tile_off[blockIdx.x];
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        u32 v = (i < n) ? in(i) : Op::id();
+        u32 agg;
+        u32 inc = block_incl<Op>(v, lds, &agg);
+        if (i < n) {
+            if (INCLUSIVE) out(i, Op::op(run, inc), v);
+            else {
+                // exclusive prefix for a sum scan: inclusive minus own value
+                out(i, run + inc - v, v);
+            }
+        }
+        run = Op::op(run, agg);
+    }
+}
+
+// Exclusive sum scan; d_total (device, may be null) receives the sum.
+template <class In, class Out>
+inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
+    if (n == 0) {
+        if (d_total) d.zero(d_total, sizeof(u32));
+        return;
+    }
+    Arena::Mark mk = d.a->mark();
+    u32 nt = ceil_div(n, SCAN_TILE);
+    u32 *ta = d.a->get<u32>(nt);
+    k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
+    PSK_LAUNCH_CHECK();
+    k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, d_total);
+    PSK_LAUNCH_CHECK();
+    k_scan_apply<OpSum, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta);
+    PSK_LAUNCH_CHECK();
+    d.a->reset(mk);
+}
+
+// Inclusive max scan.
+template <class In, class Out>
+inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
+    if (n == 0) return;
+    Arena::Mark mk = d.a->mark();
+    u32 nt = ceil_div(n, SCAN_TILE);
+    u32 *ta = d.a->get<u32>(nt);
+    k_scan_reduce<OpMax><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
+    PSK_LAUNCH_CHECK();
+    k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ta, nt, nullptr);
+    PSK_LAUNCH_CHECK();
+    k_scan_apply<OpMax, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta);
+    PSK_LAUNCH_CHECK();
+    d.a->reset(mk);
+}
+
+// ---------------------------------------------------------------------------
+// Stable 8-bit multi-split.  For i in [0, n): dig(i, &digit) returns whether
+// item i takes part and its digit (< 256).  Phase 1 counts digits per tile,
+// phase 2 scans the digit-major count matrix, phase 3 calls
+// put(i, digit, gpos, dstart) where gpos is the item's position in the
+// stable order by digit and dstart the first position of its digit, so
+// gpos - dstart is its stable rank among items of that digit.
+// ---------------------------------------------------------------------------
+constexpr int MS_ROUNDS = 16;
+constexpr u64 MS_TILE = (u64)BLOCK * MS_ROUNDS;
+
+template <class Dig>
+__global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
+    __shared__ u32 h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * MS_TILE;
+    for (int r = 0; r < MS_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        u32 key = 0;
+        bool v = (i < n) && dig(i, &key);
+        u64 peers = match_key8(v, key);
+        if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[key], (u32)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(u64)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class Dig, class Put>
+__global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
+    __shared__ u32 run[256];
+    __shared__ u32 dst0[256];
+    __shared__ u32 wc[NWAVES][256];
+    run[threadIdx.x] = off[(u64)threadIdx.x * nt + blockIdx.x];
+    dst0[threadIdx.x] = off[(u64)threadIdx.x * nt];
+    #pragma unroll
+    for (int w = 0; w < NWAVES; w++) wc[w][threadIdx.x] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * MS_TILE;
+    const int w = wave_id();
+    for (int r = 0; r < MS_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        u32 key = 0;
+        bool v = (i < n) && dig(i, &key);
+        u64 peers = match_key8(v, key);
+        u32 rk = (u32)__popcll(peers & lanemask_lt());
+        if (v && rk == 0) wc[w][key] = (u32)__popcll(peers);
+        __syncthreads();
+        if (v) {
+            u32 pos = run[key] + rk;
+            for (int w2 = 0; w2 < w; w2++) pos += wc[w2][key];
+            put(i, key, pos, dst0[key]);
+        }
+        __syncthreads();
+        {
+            u32 s = 0;
+            #pragma unroll
+            for (int w2 = 0; w2 < NWAVES; w2++) { s += wc[w2][threadIdx.x]; wc[w2][threadIdx.x] = 0; }
+            run[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// Runs the three phases; if d_counts is non-null it receives the per-digit
+// totals (256 u32) on the device.  dig_count is used by the counting phase
+// and dig by the scatter phase; they must agree (the counting phase may
+// cache its digits for the scatter phase to read back).
+template <class DigC, class Dig, class Put>
+inline void multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_counts) {
+    if (n == 0) {
+        if (d_counts) d.zero(d_counts, 256 * sizeof(u32));
+        return;
+    }
+    Arena::Mark mk = d.a->mark();
+    u32 nt = ceil_div(n, MS_TILE);
+    u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
+    k_ms_hist<<<nt, BLOCK, 0, d.s>>>(dig_count, n, hist, nt);
+    PSK_LAUNCH_CHECK();
+    u64 total = (u64)256 * nt;
+    scan_exclusive(
+        d, total, [=] __device__(u64 i) { return hist[i]; },
+        [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
+    if (d_counts) {
+        // per-digit totals from the scanned matrix: start of d+1 minus start of d
+        u32 *dc = d_counts;
+        for_each(d, 256, [=] __device__(u64 k) {
+            dc[k] = hist[(k + 1) * nt] - hist[k * nt];
+        });
+    }
+    k_ms_scatter<<<nt, BLOCK, 0, d.s>>>(dig, put, n, hist, nt);
+    PSK_LAUNCH_CHECK();
+    d.a->reset(mk);
+}
+
+template <class Dig, class Put>
+inline void multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts) {
+    multisplit2(d, n, dig, dig, put, d_counts);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (u64 key, u32 value) pairs on the low `bits` key bits.
+// Returns true if the sorted result ended in (kt, vt) rather than (k, v).
+// ---------------------------------------------------------------------------
+inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int bits) {
+    bool swapped = false;
+    for (int shift = 0; shift < bits; shift += 8) {
+        const u64 *ki = swapped ? kt : k;
+        const u32 *vi = swapped ? vt : v;
+        u64 *ko = swapped ? k : kt;
+        u32 *vo = swapped ? v : vt;
+        const int sh = shift;
+        multisplit(
+            d, n,
+            [=] __device__(u64 i, u32 *key) {
+                *key = (u32)(ki[i] >> sh) & 255u;
+                return true;
+            },
+            [=] __device__(u64 i, u32, u32 gpos, u32) {
+                ko[gpos] = ki[i];
+                vo[gpos] = vi[i];
+            },
+            nullptr);
+        swapped = !swapped;
+    }
+    return swapped;
+}
+
+inline int bit_width(u64 x) {
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b;
+}
+
+}  // namespace psk

@@ -1,0 +1,275 @@
+// psk_types.hpp -- L/S typing and LMS extraction on the GPU.
+//
+// Reference semantics (common.rs:43-68): T[n-1] is L-type; position i is
+// S-type iff T[i] < T[i+1], or T[i] == T[i+1] and i+1 is S-type; i is LMS iff
+// it is S-type, i > 0 and i-1 is L-type.
+//
+// GPU formulation: a position is "decided" when T[i] != T[i+1] (its type is
+// T[i] < T[i+1]); an undecided position takes the type of the nearest decided
+// position to its right.  Each thread owns TY_WPT 32-position words and
+// builds (decided, value) bit masks; the carry into a word is the type of the
+// first position of the next word, resolved with wave ballots inside a
+// workgroup and with a right-to-left pass over workgroup aggregates.
+// Output: S-type and LMS bit vectors (bit b of word w = position 32w+b).
+#pragma once
+
+#include "psk_prim.hpp"
+
+namespace psk {
+
+constexpr int TY_WPT = 4;
+constexpr u64 TY_POS_PER_BLOCK = (u64)BLOCK * TY_WPT * 32;
+
+template <class C>
+__device__ __forceinline__ void type_masks(const C *t, u64 n, u64 p0, u32 &D, u32 &V) {
+    D = 0;
+    V = 0;
+    C a = (p0 < n) ? t[p0] : (C)0;
+    #pragma unroll 8
+    for (int b = 0; b < 32; b++) {
+        u64 p = p0 + b;
+        if (p + 1 < n) {
+            C c = t[p + 1];
+            if (a != c) {
+                D |= 1u << b;
+                V |= (u32)(a < c) << b;
+            }
+            a = c;
+        } else {
+            D |= 1u << b;  // p == n-1 is L-type; p >= n is padding (L)
+        }
+    }
+}
+
+// Type bits of one word given the type of the position right after it.
+__device__ __forceinline__ u32 resolve_word(u32 D, u32 V, u32 carry) {
+    u32 S = 0, cur = carry & 1u;
+    #pragma unroll
+    for (int b = 31; b >= 0; b--) {
+        if ((D >> b) & 1u) cur = (V >> b) & 1u;
+        S |= cur << b;
+    }
+    return S;
+}
+
+// For every thread: the `first` value of the nearest higher-numbered thread of
+// the workgroup whose `has` is set, or block_carry if there is none.
+__device__ __forceinline__ u32 carry_from_right(bool has, u32 first, u32 block_carry, u32 *lh,
+                                                u32 *lf) {
+    const int l = lane_id(), w = wave_id();
+    u64 m = ballot(has);
+    u32 wf = __shfl(first, m ? (__ffsll((long long)m) - 1) : 0, WAVE);
+    if (l == 0) {
+        lh[w] = m != 0;
+        lf[w] = wf;
+    }
+    __syncthreads();
+    u32 wc = block_carry;
+    for (int w2 = NWAVES - 1; w2 > w; w2--)
+        if (lh[w2]) wc = lf[w2];
+    u64 higher = (l == WAVE - 1) ? 0ull : (m & ~((2ull << l) - 1ull));
+    int src = higher ? (__ffsll((long long)higher) - 1) : 0;
+    u32 f = __shfl(first, src, WAVE);
+    __syncthreads();
+    return higher ? f : wc;
+}
+
+template <class C>
+__global__ void k_type_agg(const C *t, u64 n, u8 *blk_has, u8 *blk_first) {
+    __shared__ u32 lh[NWAVES], lf[NWAVES];
+    u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
+    bool has = false;
+    u32 first = 0;
+    for (int k = 0; k < TY_WPT; k++) {
+        u32 D, V;
+        type_masks(t, n, (w0 + k) * 32, D, V);
+        if (!has && D) {
+            has = true;
+            first = (V >> (__ffs(D) - 1)) & 1u;
+        }
+    }
+    u64 m = ballot(has);
+    u32 f = __shfl(first, m ? (__ffsll((long long)m) - 1) : 0, WAVE);
+    if (lane_id() == 0) {
+        lh[wave_id()] = m != 0;
+        lf[wave_id()] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u8 bh = 0, bf = 0;
+        for (int w = 0; w < NWAVES; w++)
+            if (lh[w]) { bh = 1; bf = (u8)lf[w]; break; }
+        blk_has[blockIdx.x] = bh;
+        blk_first[blockIdx.x] = bf;
+    }
+}
+
+// One workgroup: carry[b] = type of the first position of block b+1 (0 for
+// the last block), from the block aggregates, right to left.
+constexpr int TYC_IPT = 16;
+__global__ void k_type_carry(const u8 *blk_has, const u8 *blk_first, u32 nb, u8 *carry) {
+    __shared__ u32 lh[NWAVES], lf[NWAVES];
+    __shared__ u32 chunk_carry;
+    if (threadIdx.x == 0) chunk_carry = 0;
+    __syncthreads();
+    const u64 CH = (u64)BLOCK * TYC_IPT;
+    u64 nchunks = (nb + CH - 1) / CH;
+    for (u64 ci = nchunks; ci-- > 0;) {
+        u64 b0 = ci * CH + (u64)threadIdx.x * TYC_IPT;
+        bool has = false;
+        u32 first = 0;
+        for (int k = 0; k < TYC_IPT; k++) {
+            u64 b = b0 + k;
+            if (b < nb && blk_has[b]) { has = true; first = blk_first[b]; break; }
+        }
+        u32 cr = chunk_carry;
+        u32 c = carry_from_right(has, first, cr, lh, lf);
+        for (int k = TYC_IPT - 1; k >= 0; k--) {
+            u64 b = b0 + k;
+            if (b < nb) {
+                carry[b] = (u8)c;
+                if (blk_has[b]) c = blk_first[b];
+            }
+        }
+        // aggregate of this chunk for the chunk on its left: lowest thread with has
+        u64 m = ballot(has);
+        u32 wf = __shfl(first, m ? (__ffsll((long long)m) - 1) : 0, WAVE);
+        if (lane_id() == 0) {
+            lh[wave_id()] = m != 0;
+            lf[wave_id()] = wf;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 0; w < NWAVES; w++)
+                if (lh[w]) { chunk_carry = lf[w]; break; }
+        }
+        __syncthreads();
+    }
+}
+
+template <class C>
+__global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits, u64 nwords) {
+    __shared__ u32 lh[NWAVES], lf[NWAVES];
+    u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
+    u32 D[TY_WPT], V[TY_WPT];
+    bool has = false;
+    u32 first = 0;
+    #pragma unroll
+    for (int k = 0; k < TY_WPT; k++) {
+        type_masks(t, n, (w0 + k) * 32, D[k], V[k]);
+        if (!has && D[k]) {
+            has = true;
+            first = (V[k] >> (__ffs(D[k]) - 1)) & 1u;
+        }
+    }
+    u32 c = carry_from_right(has, first, blk_carry[blockIdx.x], lh, lf);
+    #pragma unroll
+    for (int k = TY_WPT - 1; k >= 0; k--) {
+        u32 S = resolve_word(D[k], V[k], c);
+        c = S & 1u;
+        if (w0 + k < nwords) sbits[w0 + k] = S;
+    }
+}
+
+// S-type bit vector of t[0..n) into sbits[0 .. ceil(n/32)).
+template <class C>
+inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits) {
+    if (n == 0) return;
+    u64 nwords = (n + 31) / 32;
+    u32 nb = ceil_div(nwords, (u64)BLOCK * TY_WPT);
+    Arena::Mark mk = d.a->mark();
+    u8 *bh = d.a->get<u8>(nb);
+    u8 *bf = d.a->get<u8>(nb);
+    u8 *bc = d.a->get<u8>(nb);
+    k_type_agg<C><<<nb, BLOCK, 0, d.s>>>(t, n, bh, bf);
+    PSK_LAUNCH_CHECK();
+    k_type_carry<<<1, BLOCK, 0, d.s>>>(bh, bf, nb, bc);
+    PSK_LAUNCH_CHECK();
+    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords);
+    PSK_LAUNCH_CHECK();
+    d.a->reset(mk);
+}
+
+__device__ __forceinline__ u32 getbit(const u32 *bits, u64 i) { return (bits[i >> 5] >> (i & 31)) & 1u; }
+
+__device__ __forceinline__ u32 lms_word(const u32 *sbits, u64 w) {
+    u32 S = sbits[w];
+    u32 prev = w ? sbits[w - 1] : 0xFFFFFFFFu;  // position 0 is never LMS
+    return S & ~((S << 1) | (prev >> 31));
+}
+
+// Level-0 (byte) LMS bits + bucket statistics.  Per byte value c:
+// cnt[c] occurrences, nL[c] L-type occurrences, nLMS[c] LMS occurrences;
+// *m1 = number of LMS positions.  (psacak8.rs:355-374 Buckets::new)
+constexpr u64 H_F = 21;  // bits per packed LDS counter field
+__global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, u64 nwords,
+                            u32 *cnt, u32 *nL, u32 *nLMS, u32 *m1) {
+    __shared__ unsigned long long h[256];
+    __shared__ u32 red[NWAVES + 1];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    u32 lcount = 0;
+    if (w < nwords) {
+        u32 S = sbits[w];
+        u32 lms = lms_word(sbits, w);
+        lmsbits[w] = lms;
+        lcount = __popc(lms);
+        u64 p0 = w * 32;
+        u32 lim = (u32)((n - p0) < 32 ? (n - p0) : 32);
+        for (u32 b = 0; b < lim; b++) {
+            u32 c = t[p0 + b];
+            unsigned long long v = 1ull | ((unsigned long long)(~(S >> b) & 1u) << H_F) |
+                                   ((unsigned long long)((lms >> b) & 1u) << (2 * H_F));
+            atomicAdd(&h[c], v);
+        }
+    }
+    u32 tot;
+    block_exclusive_sum(lcount, red, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(m1, tot);
+    __syncthreads();
+    unsigned long long v = h[threadIdx.x];
+    if (v) {
+        const unsigned long long M = (1ull << H_F) - 1;
+        atomicAdd(&cnt[threadIdx.x], (u32)(v & M));
+        u32 l = (u32)((v >> H_F) & M), s = (u32)((v >> (2 * H_F)) & M);
+        if (l) atomicAdd(&nL[threadIdx.x], l);
+        if (s) atomicAdd(&nLMS[threadIdx.x], s);
+    }
+}
+
+// LMS bits for a u32 level; *m1 += number of LMS positions.
+__global__ void k_lms32(const u32 *sbits, u32 *lmsbits, u64 nwords, u32 *m1) {
+    __shared__ u32 red[NWAVES + 1];
+    u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    u32 lcount = 0;
+    if (w < nwords) {
+        u32 lms = lms_word(sbits, w);
+        lmsbits[w] = lms;
+        lcount = __popc(lms);
+    }
+    u32 tot;
+    block_exclusive_sum(lcount, red, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(m1, tot);
+}
+
+// Positions of set bits (ascending) -> out[0..count); if R is non-null,
+// R[p/2] = rank of p (LMS positions are >= 2 apart, so p/2 is unique).
+inline void bits_to_positions(Dev &d, const u32 *bits, u64 nwords, u32 *out, u32 *R) {
+    scan_exclusive(
+        d, nwords, [=] __device__(u64 w) { return (u32)__popc(bits[w]); },
+        [=] __device__(u64 w, u32 off, u32 v) {
+            u32 x = bits[w];
+            while (x) {
+                int b = __ffs(x) - 1;
+                x &= x - 1;
+                u32 p = (u32)(w * 32 + b);
+                out[off] = p;
+                if (R) R[p >> 1] = off;
+                off++;
+            }
+        },
+        nullptr);
+}
+
+}  // namespace psk

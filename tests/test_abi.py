@@ -1,0 +1,92 @@
+"""The C ABI without a GPU: the library loads, exports every symbol that
+include/psacak.h declares, and rejects bad arguments with status codes before
+touching the device (the crate panics on the same inputs: lib.rs:65,78-79,
+psacak32.rs:25)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import psacak_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "psacak.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(psk_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", P.lib_path()], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (psk_[a-z0-9_]+)", out))
+    decl = declared_symbols()
+    assert len(decl) >= 15
+    missing = [s for s in decl if s not in exported]
+    assert not missing, missing
+    L = ctypes.CDLL(P.lib_path())
+    for s in decl:
+        getattr(L, s)
+
+
+def test_version_and_strerror():
+    assert "gfx950" in P.version()
+    for code in range(-8, 1):
+        assert P._strerror(code)
+
+
+def test_argument_checks_without_gpu():
+    L = P._lib
+    buf = np.zeros(16, dtype=np.uint32)
+    txt = np.zeros(16, dtype=np.uint8)
+    # sa_len < n (lib.rs:78)
+    assert L.psk_psacak_inplace(None, txt.ctypes.data, 16, buf.ctypes.data, 15) == P.PSK_EINVAL
+    # n >= u32::MAX (lib.rs:65,79)
+    assert L.psk_psacak_inplace(None, txt.ctypes.data, (1 << 32) - 1, buf.ctypes.data,
+                                1 << 40) == P.PSK_ETOOLONG
+    # null pointers
+    assert L.psk_psacak_inplace(None, None, 4, buf.ctypes.data, 4) == P.PSK_EINVAL
+    assert L.psk_psacak(None, txt.ctypes.data, 4, None) == P.PSK_EINVAL
+    # psacak32 length limit (psacak32.rs:25)
+    assert L.psk_psacak32(None, buf.ctypes.data, 1 << 31, buf.ctypes.data, 1 << 31) == \
+        P.PSK_ETOOLONG
+    assert L.psk_psacak32_raw(None, buf.ctypes.data, 8, buf.ctypes.data, 7) == P.PSK_EINVAL
+    # empty texts are fine and need no device
+    assert L.psk_psacak_inplace(None, None, 0, None, 0) == P.PSK_OK
+    assert P.psacak(b"").tolist() == []
+    assert P.psacak32(np.zeros(0, dtype=np.uint32)).tolist() == []
+    with pytest.raises(P.PskError) as e:
+        P.psacak_inplace(b"abc", np.zeros(2, dtype=np.uint32))
+    assert e.value.code == P.PSK_EINVAL
+    with pytest.raises(TypeError):
+        P.psacak_inplace(b"abc", np.zeros(3, dtype=np.int64))
+    # batch: one bad member rejects the whole call before any work
+    with pytest.raises(P.PskError):
+        P._raise(L.psk_batch8(None, None, None, None, 3))
+
+
+def test_workspace_footprint_is_monotone():
+    prev = 0
+    for n in [0, 10, 1 << 10, 1 << 20, 1 << 26, 1 << 30]:
+        b = P.workspace_bytes8(n)
+        assert b >= prev
+        prev = b
+    assert P.workspace_bytes8(1 << 30) < 200 * (1 << 30)
+
+
+def test_context_create_without_device():
+    with P.Context(hbm_budget=1 << 30) as c:
+        assert c.last_error() == ""
+        assert c.last_levels().shape[1] == 6
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_no_device_is_an_error_not_an_abort():
+    with pytest.raises(P.PskError) as e:
+        P.psacak(b"mississippi")
+    assert e.value.code == P.PSK_EDEVICE

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+crate's known answers.  The SA is unique, so equality with the oracle is
+equality with the crate.  Integer work: the bar is bit-exact."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import psacak_amd as P
+from psacak_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat.json")))
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = P.Context()
+    yield c
+    c.close()
+
+
+def test_doc_tests_mississippi(ctx):
+    note("mississippi")
+    d = GOLD["doc_tests"]
+    text = d["text"].encode()
+    assert P.psacak(text, ctx).tolist() == d["psacak"]
+    suf = np.zeros(len(text), dtype=np.uint32)
+    P.psacak_inplace(text, suf, ctx)
+    assert suf.tolist() == d["psacak_inplace"]
+    suf = np.zeros(len(text) + 1, dtype=np.uint32)
+    suf[0] = len(text)
+    P.psacak_inplace(text, suf[1:], ctx)
+    assert suf.tolist() == d["with_sentinel"]
+
+
+def test_inplace_leaves_tail_untouched(ctx):
+    note("inplace tail")
+    suf = np.full(20, 7777, dtype=np.uint32)
+    P.psacak_inplace(b"banana", suf, ctx)
+    assert suf[:6].tolist() == [5, 3, 1, 0, 4, 2]
+    assert (suf[6:] == 7777).all()
+
+
+@pytest.mark.parametrize("i", range(11))
+def test_table_texts(ctx, i):
+    note(f"table {i}")
+    case = GOLD["table"][i]
+    assert P.psacak(np.array(case["text"], dtype=np.uint8), ctx).tolist() == case["sa"]
+    assert P.psacak32_raw(np.array(case["text"], dtype=np.uint32), ctx).tolist() == case["sa"]
+    tr = oracle.translate32(np.array(case["text"], dtype=np.uint32))
+    if tr is not None:
+        assert P.psacak32(tr, ctx).tolist() == case["sa"]
+
+
+def test_tiny_and_empty(ctx):
+    note("tiny")
+    for t in [b"", b"a", b"ba", b"ab", b"aa", b"cba", b"aaa", b"aba", b"abca", b"zzzz"]:
+        assert P.psacak(t, ctx).tolist() == oracle.saca_naive8(t).tolist(), t
+
+
+def test_random_small_many(ctx):
+    rng = np.random.default_rng(1234)
+    for it in range(400):
+        n = int(rng.integers(4, 600))
+        sigma = int(rng.choice([1, 2, 3, 4, 5, 16, 256]))
+        t = rng.integers(0, sigma, n).astype(np.uint8)
+        if it % 5 == 0:
+            t = t + np.uint8(60)
+        note(f"small it={it} n={n} sigma={sigma}")
+        got = P.psacak(t, ctx)
+        exp = oracle.sacak8(t)
+        assert np.array_equal(got, exp), (it, n, sigma, t.tolist())
+
+
+ADVERSARIAL = {
+    "all_same_4k": b"a" * 4096,
+    "all_same_70k": b"q" * 70000,
+    "period2": b"ab" * 30000,
+    "period3": b"abc" * 20000,
+    "descending": bytes(range(255, -1, -1)) * 64,
+    "ascending": bytes(range(256)) * 64,
+    "runs": b"".join(bytes([65 + (i % 4)]) * (1 + (i * 7919) % 300) for i in range(2000)),
+    "zero_runs": b"\x00" * 10000 + b"\x01" + b"\x00" * 9999,
+    "ff00": b"\xff\x00" * 20000,
+}
+
+
+@pytest.mark.parametrize("name", sorted(ADVERSARIAL))
+def test_adversarial(ctx, name):
+    note(f"adversarial {name}")
+    t = ADVERSARIAL[name]
+    assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
+
+
+def test_fibonacci_word(ctx):
+    note("fibonacci")
+    a, b = b"a", b"ab"
+    while len(b) < 200000:
+        a, b = b, b + a
+    assert np.array_equal(P.psacak(b, ctx), oracle.sacak8(b))
+
+
+@pytest.mark.parametrize("n", [1 << 10, 1 << 16, 1 << 20, 1 << 22])
+def test_iid_dna_sizes(ctx, n):
+    note(f"iid dna n={n}")
+    t = gen.iid_dna(n, seed=n)
+    assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
+
+
+@pytest.mark.parametrize("n", [1 << 16, 1 << 22])
+def test_markov_dna(ctx, n):
+    note(f"markov dna n={n}")
+    t = gen.markov_dna(n, seed=7)
+    assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
+
+
+@pytest.mark.parametrize("alphabet", [2, 4, 128, 256])
+def test_uniform_alphabets(ctx, alphabet):
+    note(f"uniform alphabet={alphabet}")
+    t = gen.uniform8(1 << 20, alphabet=alphabet, seed=alphabet)
+    assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
+
+
+def test_psacak32_random_bucket_pointer_texts(ctx):
+    rng = np.random.default_rng(99)
+    for it in range(150):
+        n = int(rng.integers(2, 400))
+        u = rng.integers(0, int(rng.choice([2, 3, 10, 50, 1000])), n).astype(np.uint32)
+        note(f"psacak32 it={it} n={n}")
+        tr = oracle.translate32(u)
+        if tr is None:
+            continue
+        assert np.array_equal(P.psacak32(tr, ctx), oracle.sacak32(tr)), (it, u.tolist())
+        assert np.array_equal(P.psacak32_raw(u, ctx), oracle.sacak32(tr))
+
+
+def test_psacak32_raw_zipf(ctx):
+    note("psacak32_raw zipf 1M")
+    z = gen.zipf32(1 << 20, sigma=1 << 16, s=1.1, repeat_frac=0.2, seed=5)
+    assert np.array_equal(P.psacak32_raw(z, ctx), oracle.sacak32_raw(z))
+    note("psacak32_raw all distinct")
+    perm = np.random.default_rng(1).permutation(5000).astype(np.uint32)
+    assert np.array_equal(P.psacak32_raw(perm, ctx), np.argsort(perm, kind="stable").astype(np.uint32))
+
+
+def test_psacak32_precondition(ctx):
+    note("psacak32 precondition")
+    with pytest.raises(P.PskError) as e:
+        P.psacak32(np.array([5, 1, 0, 2], dtype=np.uint32), ctx)
+    assert e.value.code == P.PSK_EPRECOND
+
+
+def test_device_api_and_checker(ctx):
+    import torch
+    note("device api")
+    t = gen.markov_dna(1 << 20, seed=3)
+    dt = torch.from_numpy(t).to("cuda:0")
+    ds = torch.empty(len(t), dtype=torch.int32, device="cuda:0")
+    P.psacak8_dev(dt, ds, ctx=ctx)
+    torch.cuda.synchronize()
+    got = ds.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, oracle.sacak8(t))
+    assert P.check8_dev(dt, ds, ctx=ctx)
+    bad = ds.clone()
+    bad[[100, 101]] = bad[[101, 100]]
+    assert not P.check8_dev(dt, bad, ctx=ctx)
+    bad = ds.clone()
+    bad[7] = bad[8]
+    assert not P.check8_dev(dt, bad, ctx=ctx)
+    lv = ctx.last_levels()
+    assert lv.shape[0] >= 2 and lv[0, 0] == len(t)
+
+
+def test_batch(ctx):
+    note("batch8")
+    L = gen.lengths(40, 4, 20000, seed=8)
+    texts = [gen.iid_dna(int(n), seed=i) for i, n in enumerate(L)]
+    outs = P.batch8(texts, ctx)
+    for t, o in zip(texts, outs):
+        assert np.array_equal(o, oracle.sacak8(t))
+
+
+def test_budget_is_enforced():
+    note("budget")
+    with P.Context(hbm_budget=1 << 20) as c:
+        with pytest.raises(P.PskError) as e:
+            P.psacak(gen.iid_dna(1 << 22), c)
+        assert e.value.code in (P.PSK_EBUDGET, P.PSK_ENOMEM)
