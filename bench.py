@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: plain GPU psacak8 on a 1 GiB DNA-like text (one MI355X).
+
+Workload (default ``--config markov1g``): 2^30 bytes, order-3 Markov chain over
+ACGT with Dirichlet(0.5) transition rows, then 10% of positions overwritten by
+copies of earlier segments (length log-uniform 1 KiB-100 KiB, 1% point
+substitutions); seeded, synthetic.  One step = one psacak8 call on the text
+already resident in HBM, producing the u32 SA in HBM.
+
+Prints ONE JSON line (rank 0).  ``value`` is input MB/s (n / step time).
+Extra fields: ``roofline`` (dominant kernel, HIP-event timed inside the
+timed region), ``cpu_baseline`` (the CPU oracle on a bounded prefix sample,
+1 core), ``parity`` (device-side SA check + sampled adjacent suffix pairs),
+``pipeline`` (whole-pipeline algorithmic bytes / device time).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+CONFIGS = {
+    # name: (description, generator kwargs)
+    "markov1g": "plain GPU psacak8, 1 GiB order-3 Markov DNA + 10% repeats",
+    "iid64m": "plain psacak8, 64 MiB i.i.d. uniform ACGT",
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_text(config: str, n: int | None):
+    from psacak_amd import gen
+
+    if config == "markov1g":
+        n = n or (1 << 30)
+        return gen.markov_dna(n, seed=20260101, repeat_frac=0.10), n
+    if config == "iid64m":
+        n = n or (1 << 26)
+        return gen.iid_dna(n, seed=20260102), n
+    raise SystemExit(f"unknown config {config}")
+
+
+def level_bytes(rows):
+    """Algorithmic bytes of the pipeline from per-level counters.
+
+    Per level with m symbols of c bytes, m1 LMS, mL L-type, mean LMS-substring
+    length l (sum of lengths S = m1 * l):
+      B = m(38.25 + 3c) + 2c(m1 + mL) + m1*44 + c*S, plus 8*m1 at the last level.
+    """
+    total = 0.0
+    for i, r in enumerate(rows):
+        m, m1, mL, k, S, c = [float(x) for x in r]
+        total += m * (38.25 + 3 * c) + 2 * c * (m1 + mL) + m1 * 44 + c * S
+        if i == len(rows) - 1:
+            total += 8 * m1
+    return total
+
+
+def sampled_pair_check(text, sa, samples=4096, seed=7, maxlen=1 << 16):
+    """Direct comparison of sampled adjacent suffix pairs (SA[i-1] < SA[i])."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    n = len(text)
+    idx = rng.integers(1, n, size=min(samples, max(n - 1, 0)))
+    tb = text.tobytes() if hasattr(text, "tobytes") else bytes(text)
+    bad = 0
+    for i in idx:
+        a, b = int(sa[i - 1]), int(sa[i])
+        # compare in chunks until a difference (long repeats need long compares)
+        off = 0
+        while True:
+            x = tb[a + off: a + off + maxlen]
+            y = tb[b + off: b + off + maxlen]
+            if x != y:
+                m = min(len(x), len(y))
+                k = next((j for j in range(m) if x[j] != y[j]), m)
+                if k == m:
+                    ok = len(x) < len(y)
+                else:
+                    ok = x[k] < y[k]
+                break
+            if len(x) < maxlen:
+                ok = False  # identical tails: not a valid pair
+                break
+            off += maxlen
+        bad += not ok
+    return int(len(idx)), int(bad)
+
+
+def cpu_baseline(text, sample_bytes):
+    """The CPU oracle (serial restatement of psacak8) on a prefix sample."""
+    import oracle
+
+    sample = text[:sample_bytes]
+    t0 = time.perf_counter()
+    sa = oracle.sacak8(sample)
+    dt = time.perf_counter() - t0
+    assert oracle.check_sa8(sample, sa) == 0
+    return {
+        "value": round(len(sample) / dt / 1e6, 3),
+        "unit": "MB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {len(sample)} bytes of the same text, serial C restatement "
+                  f"of psacak8 (oracle/psk_oracle.c), {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="markov1g", choices=sorted(CONFIGS))
+    ap.add_argument("--n", type=int, default=0, help="override text length (testing only)")
+    ap.add_argument("--cpu-sample", type=int, default=64 << 20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import psacak_amd as P
+
+    log(f"generating {args.config}")
+    text, n = make_text(args.config, args.n or None)
+    dev = torch.device("cuda:0")
+    d_text = torch.from_numpy(text).to(dev)
+    d_sa = torch.empty(n, dtype=torch.int32, device=dev)
+    ctx = P.Context()
+    stream = torch.cuda.Stream()  # default priority
+    torch.cuda.synchronize()
+
+    log(f"warmup x{args.warmup}")
+    for _ in range(args.warmup):
+        P.psacak8_dev(d_text, d_sa, n, ctx=ctx, stream=stream)
+    torch.cuda.synchronize()
+
+    P.kernel_profile(ctx, True)
+    log(f"timing x{args.steps}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(args.steps):
+        P.psacak8_dev(d_text, d_sa, n, ctx=ctx, stream=stream)
+        dev_ms.append(float(ctx.last_timing_ms()[0]))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kstats = P.kernel_stats(ctx)
+    P.kernel_profile(ctx, False)
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    mbps = n / (ms_per_step / 1000.0) / 1e6
+    rows = ctx.last_levels()
+
+    # parity (outside the timed region)
+    log("parity check")
+    ok_dev = P.check8_dev(d_text, d_sa, n, ctx=ctx)
+    sa_host = d_sa.cpu().numpy().view(np.uint32)
+    pairs, bad_pairs = sampled_pair_check(text, sa_host)
+
+    # host-to-host rate (PCIe included): one extra call from host buffers
+    log("host-to-host call")
+    sa_h = np.empty(n, dtype=np.uint32)
+    th0 = time.perf_counter()
+    P.psacak_inplace(text, sa_h, ctx)
+    th = time.perf_counter() - th0
+    h2h_equal = bool(np.array_equal(sa_h, sa_host))
+
+    # dominant kernel roofline
+    dom = max(kstats, key=lambda k: k["ms"]) if kstats else None
+    roof = None
+    if dom and dom["launches"]:
+        avg_ms = dom["ms"] / dom["launches"]
+        bytes_per_launch = dom["bytes"] / dom["launches"]
+        achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+        roof = {
+            "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+            "bytes_per_launch": int(bytes_per_launch), "launches": dom["launches"],
+            "share_of_device_time": round(dom["ms"] / max(sum(dev_ms), 1e-9), 4),
+        }
+    alg = level_bytes(rows)
+    dev_avg = sum(dev_ms) / len(dev_ms)
+    pipeline = {
+        "algorithmic_bytes": int(alg), "device_ms": round(dev_avg, 3),
+        "achieved_GBps": round(alg / (dev_avg / 1000.0) / 1e9, 2),
+        "frac": round(alg / (dev_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+        "levels": [[int(x) for x in r] for r in rows],
+        "kernels": sorted(
+            [{"name": k["name"], "launches": k["launches"], "ms": round(k["ms"] / args.steps, 3)}
+             for k in kstats], key=lambda k: -k["ms"]),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(text, min(args.cpu_sample, n))
+
+    out = {
+        "metric": "input MB/s at 1 GiB DNA (bit-exact SA)",
+        "value": round(mbps, 2),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8->u32",
+        "data": "synthetic (seeded generator psacak_amd/csrc/gen.c)",
+        "config": {"workload": CONFIGS[args.config], "n": n, "seed": 20260101,
+                   "sa": "u32", "path": "psk_psacak8_dev, text resident in HBM"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity": {"device_check": bool(ok_dev), "sampled_pairs": pairs,
+                   "bad_pairs": bad_pairs, "host_path_equal": h2h_equal},
+        "host_to_host_MBps": round(n / th / 1e6, 2),
+        "pipeline": pipeline,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,18 @@ int psk_last_levels(const psk_ctx *ctx, uint64_t *rows, int max_levels);
  * [0] total, [1] level-0 induce, [2] deeper levels, [3] naming+typing. */
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
 
+/* ---- kernel profiling ---------------------------------------------------- */
+/* No counterpart in the reference (its CLI times whole calls, bin/psacak.rs:
+ * 40-48).  on != 0 starts recording HIP-event durations and algorithmic bytes
+ * of the tagged kernels launched by this context (and clears old totals);
+ * on == 0 stops. */
+int psk_profile(psk_ctx *ctx, int on);
+/* Totals per tagged kernel since psk_profile(ctx, 1): names are written to
+ * names ('\n'-separated, NUL-terminated, names_len bytes), and launches[i],
+ * ms[i], bytes[i] for i < max.  Returns the number of kernels. */
+int psk_profile_stats(psk_ctx *ctx, char *names, uint64_t names_len, uint64_t *launches,
+                      double *ms, double *bytes, int max);
+
 /* ---- batched ------------------------------------------------------------ */
 /* SA of count independent texts.  texts[i] (lens[i] bytes) -> sas[i].
  * Host buffers.  Small texts are packed into shared device batches. */

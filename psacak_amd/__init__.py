@@ -24,7 +24,8 @@ import numpy as np
 __all__ = [
     "PskError", "Context", "psacak", "psacak_inplace", "psacak32", "psacak32_raw",
     "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "batch8", "batch8_dev",
-    "workspace_bytes8", "workspace_bytes32", "lib_path", "version",
+    "workspace_bytes8", "workspace_bytes32", "lib_path", "version", "kernel_profile",
+    "kernel_stats",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -43,6 +44,15 @@ class PskError(RuntimeError):
 
 
 def _load():
+    # PyTorch-ROCm ships its own HIP runtime (libamdhip64, SONAME
+    # libamdhip64.so.7) and loads it by file name.  If this library's
+    # dependency on the system runtime were resolved first, torch would load
+    # a second runtime into the process and find no GPU.  Importing torch
+    # first makes both share torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(_LIB_PATH):
         raise ImportError(
             f"psacak_amd: native library {_LIB_PATH} is missing; build it with "
@@ -70,6 +80,8 @@ def _load():
         "psk_batch8": ([vp, vp, vp, vp, u64], i32),
         "psk_batch8_dev": ([vp, vp, vp, u64, vp, vp], i32),
         "psk_check8_dev": ([vp, vp, u64, vp, ctypes.POINTER(i32), vp], i32),
+        "psk_profile": ([vp, i32], i32),
+        "psk_profile_stats": ([vp, ctypes.c_char_p, u64, vp, vp, vp, i32], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -78,7 +90,19 @@ def _load():
     return L
 
 
-_lib = _load()
+class _Lib:
+    """Loads libpsacak_amd.so on first use (so ``psacak_amd.build`` can run
+    before the library exists); a missing library raises ImportError."""
+
+    _handle = None
+
+    def __getattr__(self, name):
+        if _Lib._handle is None:
+            _Lib._handle = _load()
+        return getattr(_Lib._handle, name)
+
+
+_lib = _Lib()
 
 
 def lib_path() -> str:
@@ -268,6 +292,28 @@ def check8_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
                              ctypes.byref(ok), _stream_ptr(stream))
     _raise(rc, ctx)
     return bool(ok.value)
+
+
+def kernel_profile(ctx: Context, on: bool) -> None:
+    """Start (clearing old totals) or stop HIP-event profiling of the tagged
+    kernels launched by ``ctx``."""
+    ctx._check(_lib.psk_profile(ctx.handle, 1 if on else 0))
+
+
+def kernel_stats(ctx: Context):
+    """[{name, launches, ms, bytes}] per tagged kernel since kernel_profile(ctx, True)."""
+    cap = 256
+    names = ctypes.create_string_buffer(1 << 16)
+    launches = np.zeros(cap, dtype=np.uint64)
+    ms = np.zeros(cap, dtype=np.float64)
+    by = np.zeros(cap, dtype=np.float64)
+    n = _lib.psk_profile_stats(ctx.handle, names, len(names), launches.ctypes.data,
+                               ms.ctypes.data, by.ctypes.data, cap)
+    if n < 0:
+        raise PskError(n, ctx.last_error())
+    nm = names.value.decode().split("\n") if n else []
+    return [{"name": nm[i], "launches": int(launches[i]), "ms": float(ms[i]),
+             "bytes": float(by[i])} for i in range(n)]
 
 
 def batch8(texts, ctx: Context | None = None):

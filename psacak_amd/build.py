@@ -18,6 +18,8 @@ GEN = os.path.join(HERE, "libpsk_gen.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",
+    # code object v5 loads on the ROCm 7.0 runtime bundled with PyTorch too
+    "-mcode-object-version=5",
     "-O3",
     "-std=c++17",
     "-fPIC",

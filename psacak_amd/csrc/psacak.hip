@@ -21,6 +21,7 @@ struct psk_ctx {
     u64 budget = 0;
     std::string err;
     Runtime rt;
+    Prof prof;
     double last_ms[4] = {0, 0, 0, 0};
 };
 
@@ -44,6 +45,7 @@ void ensure_ready(psk_ctx *c) {
     c->rt.d.s = c->stream;
     c->rt.d.a = &c->arena;
     c->rt.d.pinned = c->pinned;
+    c->rt.d.prof = &c->prof;
     c->ready = true;
 }
 
@@ -201,6 +203,7 @@ void psk_ctx_destroy(psk_ctx *ctx) {
         (void)hipStreamDestroy(ctx->stream);
         (void)hipHostFree(ctx->pinned);
         (void)hipFree(ctx->rt.d.err);
+        ctx->prof.release();
     }
     delete ctx;
 }
@@ -342,6 +345,38 @@ int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries) {
     int n = std::min(max_entries, 4);
     for (int i = 0; i < n; i++) ms[i] = ctx->last_ms[i];
     return n;
+}
+
+int psk_profile(psk_ctx *ctx, int on) {
+    if (!ctx) return PSK_EINVAL;
+    return guarded(ctx, [&] {
+        ensure_ready(ctx);
+        if (on) ctx->prof.clear();
+        else ctx->prof.flush();
+        ctx->prof.on = on != 0;
+    });
+}
+
+int psk_profile_stats(psk_ctx *ctx, char *names, uint64_t names_len, uint64_t *launches,
+                      double *ms, double *bytes, int max) {
+    if (!ctx || !names || names_len == 0 || max < 0) return PSK_EINVAL;
+    int count = 0;
+    int rc = guarded(ctx, [&] {
+        if (!ctx->ready) { names[0] = 0; return; }
+        const auto &t = ctx->prof.totals();
+        std::string all;
+        for (size_t i = 0; i < t.size() && (int)i < max; i++) {
+            if (i) all += "\n";
+            all += t[i].name;
+            if (launches) launches[i] = t[i].launches;
+            if (ms) ms[i] = t[i].ms;
+            if (bytes) bytes[i] = t[i].bytes;
+            count++;
+        }
+        if (all.size() + 1 > names_len) fail(PSK_EINVAL, "names buffer too small");
+        memcpy(names, all.c_str(), all.size() + 1);
+    });
+    return rc == PSK_OK ? count : rc;
 }
 
 int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,

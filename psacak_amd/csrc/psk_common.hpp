@@ -167,4 +167,96 @@ class Arena {
     u64 min_chunk_ = 64ull << 20;
 };
 
+// ---- kernel profiler ---------------------------------------------------------
+// When enabled, tagged launches are bracketed by HIP events on their stream;
+// each record carries the launch's algorithmic bytes (static, known on the
+// host) plus a device-side u64 slot that kernels with data-dependent traffic
+// add to.  Nothing synchronises until stats are collected.
+class Prof {
+  public:
+    struct Total { std::string name; u64 launches = 0; double ms = 0, bytes = 0; };
+
+    bool on = false;
+
+    ~Prof() { release(); }
+
+    // returns the record index, or -1 when profiling is off
+    int begin(hipStream_t s, const char *name, double static_bytes) {
+        if (!on) return -1;
+        if (!dyn_) {
+            if (hipMalloc((void **)&dyn_, kMaxRecs * sizeof(u64)) != hipSuccess) { on = false; return -1; }
+            if (hipMemset(dyn_, 0, kMaxRecs * sizeof(u64)) != hipSuccess) { on = false; return -1; }
+        }
+        if (recs_.size() >= kMaxRecs) flush();
+        Rec r;
+        r.name = name;
+        r.bytes = static_bytes;
+        r.a = event();
+        r.b = event();
+        r.s = s;
+        (void)hipEventRecord(r.a, s);
+        recs_.push_back(r);
+        return (int)recs_.size() - 1;
+    }
+    void end(int rec) {
+        if (rec < 0) return;
+        (void)hipEventRecord(recs_[rec].b, recs_[rec].s);
+    }
+    u64 *dyn_slot(int rec) { return rec < 0 ? nullptr : dyn_ + rec; }
+
+    // synchronise and fold the pending records into the totals
+    void flush() {
+        if (recs_.empty()) return;
+        std::vector<u64> dyn(recs_.size(), 0);
+        // the streams, not only the end events: data-dependent byte slots are
+        // filled by kernels queued after a record's end event
+        std::vector<hipStream_t> seen;
+        for (auto &r : recs_)
+            if (std::find(seen.begin(), seen.end(), r.s) == seen.end()) seen.push_back(r.s);
+        for (auto s : seen) (void)hipStreamSynchronize(s);
+        (void)hipMemcpy(dyn.data(), dyn_, recs_.size() * sizeof(u64), hipMemcpyDeviceToHost);
+        (void)hipMemset(dyn_, 0, recs_.size() * sizeof(u64));
+        for (size_t i = 0; i < recs_.size(); i++) {
+            Rec &r = recs_[i];
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            Total &t = total(r.name);
+            t.launches++;
+            t.ms += ms;
+            t.bytes += r.bytes + (double)dyn[i];
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        recs_.clear();
+    }
+    const std::vector<Total> &totals() { flush(); return totals_; }
+    void clear() { flush(); totals_.clear(); }
+    void release() {
+        recs_.clear();
+        for (auto e : pool_) (void)hipEventDestroy(e);
+        pool_.clear();
+        if (dyn_) (void)hipFree(dyn_);
+        dyn_ = nullptr;
+    }
+
+  private:
+    static constexpr size_t kMaxRecs = 8192;
+    struct Rec { std::string name; double bytes; hipEvent_t a, b; hipStream_t s; };
+    hipEvent_t event() {
+        if (!pool_.empty()) { hipEvent_t e = pool_.back(); pool_.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    Total &total(const std::string &name) {
+        for (auto &t : totals_) if (t.name == name) return t;
+        totals_.push_back(Total{name, 0, 0, 0});
+        return totals_.back();
+    }
+    std::vector<Rec> recs_;
+    std::vector<hipEvent_t> pool_;
+    std::vector<Total> totals_;
+    u64 *dyn_ = nullptr;
+};
+
 }  // namespace psk

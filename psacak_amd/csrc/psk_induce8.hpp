@@ -153,7 +153,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         const u64 R = nl + nlms;
         if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
-        multisplit2(
+        int ps = multisplit2(
             d, R,
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
@@ -177,8 +177,12 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                 if (dst < nn) SA[dst] = x - 1;
                 else atomicOr(err, 4u);
             },
-            counts);
-        for_each(d, 256, [=] __device__(u64 k) { fill[k] += counts[k]; });
+            counts, "induce8", 6.0 * R, 5.0 * R);
+        u64 *slot = d.pslot(ps);
+        for_each(d, 256, [=] __device__(u64 k) {
+            fill[k] += counts[k];
+            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 4ull * counts[k]);
+        });
     }
     // ---------------- S-scan ----------------
     d.zero(fill, 256 * sizeof(u32));
@@ -190,7 +194,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };
-        multisplit2(
+        int ps = multisplit2(
             d, R,
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
@@ -214,8 +218,12 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                 if (dst >= 0 && dst < (i64)nn) SA[dst] = x - 1;
                 else atomicOr(err, 8u);
             },
-            counts);
-        for_each(d, 256, [=] __device__(u64 k) { fill[k] += counts[k]; });
+            counts, "induce8", 6.0 * R, 5.0 * R);
+        u64 *slot = d.pslot(ps);
+        for_each(d, 256, [=] __device__(u64 k) {
+            fill[k] += counts[k];
+            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 4ull * counts[k]);
+        });
     }
 }
 

@@ -20,6 +20,11 @@ struct Dev {
     Arena *a = nullptr;
     u32 *pinned = nullptr;  // >= 4096 u32 of pinned host memory
     u32 *err = nullptr;     // device flag: an index left its range (internal error)
+    Prof *prof = nullptr;
+
+    int pbegin(const char *name, double bytes) { return prof ? prof->begin(s, name, bytes) : -1; }
+    void pend(int r) { if (prof) prof->end(r); }
+    u64 *pslot(int r) { return prof ? prof->dyn_slot(r) : nullptr; }
 
     u32 read_u32(const u32 *d) {
         PSK_HIP(hipMemcpyAsync(pinned, d, sizeof(u32), hipMemcpyDeviceToHost, s));
@@ -281,17 +286,26 @@ __global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
 // totals (256 u32) on the device.  dig_count is used by the counting phase
 // and dig by the scatter phase; they must agree (the counting phase may
 // cache its digits for the scatter phase to read back).
+// Profiling: tag != nullptr names the two kernels "<tag>_count" and
+// "<tag>_scatter" with bytes_count / bytes_scatter algorithmic bytes; the
+// scatter's record index is returned (for data-dependent byte slots).
 template <class DigC, class Dig, class Put>
-inline void multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_counts) {
+inline int multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_counts,
+                       const char *tag = nullptr, double bytes_count = 0,
+                       double bytes_scatter = 0) {
     if (n == 0) {
         if (d_counts) d.zero(d_counts, 256 * sizeof(u32));
-        return;
+        return -1;
     }
     Arena::Mark mk = d.a->mark();
     u32 nt = ceil_div(n, MS_TILE);
     u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
+    std::string tc = tag ? std::string(tag) + "_count" : std::string();
+    std::string ts = tag ? std::string(tag) + "_scatter" : std::string();
+    int pr = tag ? d.pbegin(tc.c_str(), bytes_count) : -1;
     k_ms_hist<<<nt, BLOCK, 0, d.s>>>(dig_count, n, hist, nt);
     PSK_LAUNCH_CHECK();
+    d.pend(pr);
     u64 total = (u64)256 * nt;
     scan_exclusive(
         d, total, [=] __device__(u64 i) { return hist[i]; },
@@ -303,14 +317,18 @@ inline void multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_
             dc[k] = hist[(k + 1) * nt] - hist[k * nt];
         });
     }
+    int ps = tag ? d.pbegin(ts.c_str(), bytes_scatter) : -1;
     k_ms_scatter<<<nt, BLOCK, 0, d.s>>>(dig, put, n, hist, nt);
     PSK_LAUNCH_CHECK();
+    d.pend(ps);
     d.a->reset(mk);
+    return ps;
 }
 
 template <class Dig, class Put>
-inline void multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts) {
-    multisplit2(d, n, dig, dig, put, d_counts);
+inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char *tag = nullptr,
+                      double bytes_count = 0, double bytes_scatter = 0) {
+    return multisplit2(d, n, dig, dig, put, d_counts, tag, bytes_count, bytes_scatter);
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +353,7 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
                 ko[gpos] = ki[i];
                 vo[gpos] = vi[i];
             },
-            nullptr);
+            nullptr, "radix", 8.0 * n, 24.0 * n);
         swapped = !swapped;
     }
     return swapped;
