@@ -64,7 +64,11 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 
 // Sort items[0..A) by chain keys (see above); RK holds the labels of every
 // position 0..m (m = sentinel) and receives the final labels of the items.
-inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A) {
+// If last_slot_labels, the items' initial group labels name the group's last
+// slot + 1 instead of its first (S parts are known by their tail); the first
+// round converts them, so later rounds use first-slot labels.
+inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
+                       bool last_slot_labels) {
     Dev &d = rt.d;
     if (A == 0) return;
     Arena::Mark mk = d.a->mark();
@@ -96,20 +100,23 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A) {
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
         const u64 AA = A;
+        // groups are contiguous in sorted order: record each group's first
+        // (or, for last-slot labels, last) sorted index with a plain store
+        const bool lastconv = last_slot_labels && rounds == 1;
         for_each(d, AA, [=] __device__(u64 i) {
             u64 hi = K[i] >> B;
-            if (hi <= mm) fidx[hi] = 0xFFFFFFFFu;
-            else atomicOr(err, 32u);
+            if (hi > mm) { atomicOr(err, 32u); return; }
+            bool edge = lastconv ? (i + 1 == AA || (K[i + 1] >> B) != hi)
+                                 : (i == 0 || (K[i - 1] >> B) != hi);
+            if (edge) fidx[hi] = (u32)i;
         });
-        for_each(d, AA, [=] __device__(u64 i) {
-            u64 hi = K[i] >> B;
-            if (hi <= mm) atomicMin(&fidx[hi], (u32)i);
-        });
+        // new label = first slot of the item's subgroup + 1
         scan_inclusive_max(
             d, AA, [=] __device__(u64 i) { return (i == 0 || K[i] != K[i - 1]) ? (u32)i : 0u; },
             [=] __device__(u64 i, u32 s0, u32) {
                 u64 hi = K[i] >> B;
-                if (hi <= mm) RK[V[i]] = (u32)hi + (s0 - fidx[hi]);
+                if (hi > mm) return;
+                RK[V[i]] = lastconv ? (u32)hi - (fidx[hi] - s0) : (u32)hi + (s0 - fidx[hi]);
             });
         scan_exclusive(
             d, AA,
@@ -139,14 +146,29 @@ inline void labels_L(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
     });
 }
 
-// S-scan labels: first slot of the bucket's S part + 1 = s[x] - nS + 2.
-inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK, u32 *cnt) {
-    d.zero(cnt, (m + 1) * sizeof(u32));
+// S-scan labels: last slot of the bucket's S part + 1 = s[x] + 1 (tail-1 is
+// the S-type symbol itself); chain_sort is told to convert them.
+inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
     for_each(d, m, [=] __device__(u64 x) {
-        if (getbit(sbits, x)) atomicAdd(&cnt[s[x]], 1u);
+        if (getbit(sbits, x)) RK[x] = s[x] + 1;
     });
-    for_each(d, m, [=] __device__(u64 x) {
-        if (getbit(sbits, x)) RK[x] = s[x] + 2 - cnt[s[x]];
+}
+
+// Labels of LMS suffixes placed at their bucket tails in the order of list
+// (grouped by symbol value, e.g. sorted by it): the last of a group gets slot
+// s[x] = tail-1, the one before it tail-2, ...  grp: scratch of m+1 u32.
+inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, u64 cnt, u32 *RK,
+                                u32 *grp) {
+    u32 *err = d.err;
+    for_each(d, cnt, [=] __device__(u64 j) {
+        u32 v = s[list[j]];
+        if (j + 1 == cnt || s[list[j + 1]] != v) grp[v] = (u32)j;
+    });
+    for_each(d, cnt, [=] __device__(u64 j) {
+        u32 p = list[j], v = s[p];
+        u32 back = grp[v] - (u32)j;
+        if (back <= v) RK[p] = v - back + 1;
+        else atomicOr(err, 256u);
     });
 }
 
@@ -246,19 +268,22 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     // ---- stage 1: sort LMS substrings --------------------------------------
     // LMS characters at their bucket tails, in arbitrary order (psacak32.rs:60-99)
     labels_L(d, s, m, sbits, RK);
-    d.zero(cnt, (m + 1) * sizeof(u32));
-    {
-        u32 *err = d.err;
+    if (m1) {
+        // LMS characters grouped by symbol (stable: text order inside a group)
+        Arena::Mark m2 = d.a->mark();
+        u64 *k0 = d.a->get<u64>(m1), *k1 = d.a->get<u64>(m1);
+        u32 *v0 = d.a->get<u32>(m1), *v1 = d.a->get<u32>(m1);
         for_each(d, m1, [=] __device__(u64 r) {
-            u32 x = lms_text[r];
-            u32 k = atomicAdd(&cnt[s[x]], 1u);
-            if (k <= s[x]) RK[x] = s[x] - k + 1;
-            else atomicOr(err, 128u);
+            k0[r] = s[lms_text[r]];
+            v0[r] = lms_text[r];
         });
+        bool sw = radix_sort_kv(d, m1, k0, v0, k1, v1, bit_width(m));
+        labels_lms_at_tails(d, s, sw ? v1 : v0, m1, RK, cnt);
+        d.a->reset(m2);
     }
-    chain_sort(rt, m, RK, Litems, nL);
-    labels_S(d, s, m, sbits, RK, cnt);
-    chain_sort(rt, m, RK, Sitems, nS);
+    chain_sort(rt, m, RK, Litems, nL, false);
+    labels_S(d, s, m, sbits, RK);
+    chain_sort(rt, m, RK, Sitems, nS, true);
     u32 *lms_sorted = d.a->get<u32>(m1 + 1);
     extract_lms_sorted(d, m, RK, lmsbits, SA, lms_sorted);
     if (m1) {
@@ -274,21 +299,11 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
     labels_L(d, s, m, sbits, RK);
-    if (m1) {
-        d.zero(cnt, (m + 1) * sizeof(u32));
-        const u32 *sl = SA1;
-        for_each(d, m1, [=] __device__(u64 j) { atomicMax(&cnt[s[sl[j]]], (u32)j); });
-        u32 *err = d.err;
-        for_each(d, m1, [=] __device__(u64 j) {
-            u32 p = sl[j], v = s[p];
-            u32 back = cnt[v] - (u32)j;
-            if (back <= v) RK[p] = v - back + 1;
-            else atomicOr(err, 256u);
-        });
-    }
-    chain_sort(rt, m, RK, Litems, nL);
-    labels_S(d, s, m, sbits, RK, cnt);
-    chain_sort(rt, m, RK, Sitems, nS);
+    // sorted LMS suffixes are grouped by first symbol already
+    if (m1) labels_lms_at_tails(d, s, SA1, m1, RK, cnt);
+    chain_sort(rt, m, RK, Litems, nL, false);
+    labels_S(d, s, m, sbits, RK);
+    chain_sort(rt, m, RK, Sitems, nS, true);
     write_sa_from_labels(d, m, RK, SA);
     d.a->reset(mk);
 }

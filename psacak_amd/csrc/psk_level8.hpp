@@ -159,12 +159,12 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
         [=] __device__(u64 i, u32 s0, u32) { first[i] = s0; });
     u32 *sb = d.a->get<u32>((n + 31) / 32 + 1);
     compute_types(d, t, n, sb);
-    // last sorted index of each run, keyed by the run's first index
+    // last sorted index of each run, keyed by the run's first index (runs
+    // are contiguous: a plain store at each run's end)
     u32 *last = d.a->get<u32>(n);
     for_each(d, n, [=] __device__(u64 i) {
-        if (i == 0 || K[i] != K[i - 1]) last[i] = 0;
+        if (i + 1 == nn || K[i + 1] != K[i]) last[first[i]] = (u32)i;
     });
-    for_each(d, n, [=] __device__(u64 i) { atomicMax(&last[first[i]], (u32)i); });
     for_each(d, nn, [=] __device__(u64 i) {
         u32 p = V[i];
         out[p] = getbit(sb, p) ? last[first[i]] : first[i];
