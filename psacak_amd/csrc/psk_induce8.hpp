@@ -42,31 +42,36 @@ struct QPos {
 
 constexpr u32 CHAIN_SMALL = 8192;
 
+// Every queue entry x gets its predecessor byte cached in dig[j] (j = read
+// index), gathered once: the chain step needs "T[x-1] == c" and the
+// multi-split that follows needs T[x-1] as its digit.  Suffix 0 has no
+// predecessor; consumers check x > 0 themselves.
+__device__ __forceinline__ u32 pred_byte(const u8 *T, u32 x) { return x ? (u32)T[x - 1] : 0u; }
+
 // One workgroup finishes a queue: generations until the queue is full (limit)
-// or a generation is empty.  Single items are followed by one thread.
-__global__ void k_chain_finish(const u8 *T, u32 *SA, QPos qp, u32 c, u64 gs, u64 ge, u64 limit,
-                               u32 *err) {
+// or a generation is empty.  A lone chain is followed by one thread.
+__global__ void k_chain_finish(const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c, u64 gs, u64 ge,
+                               u64 limit, u32 *err) {
     __shared__ u32 lds[NWAVES + 1];
-    while (gs < ge && ge < limit) {
+    while (gs < ge) {
         u64 cnt = ge - gs;
         if (cnt == 1) {
             // a lone chain: follow it serially
             if (threadIdx.x == 0) {
-                u32 x = SA[qp(gs)];
-                u64 e = ge;
-                while (e < limit && x > 0 && T[x - 1] == c) {
-                    SA[qp(e)] = x - 1;
+                u64 j = gs;
+                u32 x = SA[qp(j)];
+                for (;;) {
+                    u32 cp = pred_byte(T, x);
+                    dig[j] = (u8)cp;
+                    if (!(x > 0 && cp == c)) break;
+                    if (j + 1 >= limit) { atomicOr(err, 1u); break; }
+                    SA[qp(j + 1)] = x - 1;
                     x--;
-                    e++;
+                    j++;
                 }
-                lds[0] = (u32)(e - ge);
             }
             __syncthreads();
-            u64 e = ge + lds[0];
-            __syncthreads();
-            gs = e;  // everything up to e is processed
-            ge = e;
-            break;
+            return;
         }
         u64 newend = ge;
         for (u64 j0 = gs; j0 < ge; j0 += BLOCK) {
@@ -75,7 +80,9 @@ __global__ void k_chain_finish(const u8 *T, u32 *SA, QPos qp, u32 c, u64 gs, u64
             bool v = false;
             if (j < ge) {
                 x = SA[qp(j)];
-                v = x > 0 && T[x - 1] == c;
+                u32 cp = pred_byte(T, x);
+                dig[j] = (u8)cp;
+                v = x > 0 && cp == c;
             }
             u32 tot;
             u32 ex = block_exclusive_sum((u32)v, lds, &tot);
@@ -92,25 +99,28 @@ __global__ void k_chain_finish(const u8 *T, u32 *SA, QPos qp, u32 c, u64 gs, u64
 }
 
 // Fill one bucket's queue: positions qp(0..limit); generation 0 is
-// [0, *d_start).  d_tmp: one device u32.
-inline void chain_queue8(Dev &d, const u8 *T, u32 *SA, QPos qp, u32 c, const u32 *d_start,
-                         u64 limit, u32 *d_tmp) {
+// [0, *d_start).  Caches every entry's predecessor byte in dig[0..limit).
+// d_tmp: one device u32.
+inline void chain_queue8(Dev &d, const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c,
+                         const u32 *d_start, u64 limit, u32 *d_tmp) {
     u64 gs = 0, ge = d.read_u32(d_start);
-    while (ge > gs && ge < limit) {
+    u32 *err = d.err;
+    while (ge > gs) {
         u64 cnt = ge - gs;
         if (cnt <= CHAIN_SMALL) {
-            k_chain_finish<<<1, BLOCK, 0, d.s>>>(T, SA, qp, c, gs, ge, limit, d.err);
+            k_chain_finish<<<1, BLOCK, 0, d.s>>>(T, SA, dig, qp, c, gs, ge, limit, err);
             PSK_LAUNCH_CHECK();
             return;
         }
         const u64 g0 = gs, e0 = ge, lim = limit;
-        u32 *err = d.err;
+        // gather the predecessor bytes of this generation (the only gather)
+        int pr = d.pbegin("chain8_gather", 9.0 * cnt);
+        for_each(d, cnt, [=] __device__(u64 i) { dig[g0 + i] = (u8)pred_byte(T, SA[qp(g0 + i)]); });
+        d.pend(pr);
+        if (ge >= limit) return;  // queue full: this was the last generation
         scan_exclusive(
             d, cnt,
-            [=] __device__(u64 i) {
-                u32 x = SA[qp(g0 + i)];
-                return (u32)(x > 0 && T[x - 1] == c);
-            },
+            [=] __device__(u64 i) { return (u32)(dig[g0 + i] == c && SA[qp(g0 + i)] > 0); },
             [=] __device__(u64 i, u32 off, u32 v) {
                 if (v) {
                     if (e0 + off < lim) SA[qp(e0 + off)] = SA[qp(g0 + i)] - 1;
@@ -149,7 +159,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], nlms = B.nLMS[c];
         const u32 cc = (u32)c;
-        if (nl) chain_queue8(d, T, SA, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
+        if (nl) chain_queue8(d, T, SA, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
         const u64 R = nl + nlms;
         if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
@@ -157,11 +167,9 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
             d, R,
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
-                u32 k = 0;
-                if (x > 0) {
-                    u32 cp = T[x - 1];
-                    if (cp > cc) k = cp;
-                }
+                // the queue part has its predecessor bytes cached
+                u32 cp = j < nl ? (u32)dig[j] : pred_byte(T, x);
+                u32 k = (x > 0 && cp > cc) ? cp : 0u;
                 dig[j] = (u8)k;  // 0 = not induced (valid digits are > c >= 0)
                 *key = k;
                 return k != 0;
@@ -190,7 +198,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
         const u32 cc = (u32)c;
-        if (ns) chain_queue8(d, T, SA, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
+        if (ns) chain_queue8(d, T, SA, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };
@@ -198,11 +206,8 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
             d, R,
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
-                u32 k = 255;
-                if (x > 0) {
-                    u32 cp = T[x - 1];
-                    if (cp < cc) k = cp;
-                }
+                u32 cp = j < ns ? (u32)dig[j] : pred_byte(T, x);
+                u32 k = (x > 0 && cp < cc) ? cp : 255u;
                 dig[j] = (u8)k;  // 255 = not induced (valid digits are < c <= 255)
                 *key = k;
                 return k != 255;

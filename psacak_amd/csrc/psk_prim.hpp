@@ -332,30 +332,124 @@ inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort of (u64 key, u32 value) pairs on the low `bits` key bits.
+// LSD radix sort of (u64 key, u32 value) pairs on the low `bits` key bits,
+// 8-bit digits.  Per pass: a per-tile digit count, one scan of the
+// digit-major count matrix, and a scatter that first reorders its tile by
+// digit in LDS (stable), so each digit's items leave the tile as one
+// contiguous, coalesced run.
 // Returns true if the sorted result ended in (kt, vt) rather than (k, v).
 // ---------------------------------------------------------------------------
+constexpr int RS_ROUNDS = 8;
+constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
+
+__global__ void k_rs_count(const u64 *__restrict__ ki, u64 n, int shift, u32 *hist, u32 nt) {
+    __shared__ u32 h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * RS_TILE;
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        bool v = i < n;
+        u32 key = v ? (u32)(ki[i] >> shift) & 255u : 0u;
+        u64 peers = match_key8(v, key);
+        if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[key], (u32)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(u64)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
+}
+
+// off: exclusive scan of the digit-major count matrix (256*nt + 1 entries,
+// the last one the total), so the count of (digit d, tile t) is
+// off[d*nt+t+1] - off[d*nt+t].
+__global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki,
+                                                      const u32 *__restrict__ vi,
+                                                      u64 *__restrict__ ko, u32 *__restrict__ vo,
+                                                      u64 n, int shift, const u32 *off, u32 nt) {
+    __shared__ u64 sk[RS_TILE];
+    __shared__ u32 sv[RS_TILE];
+    __shared__ u32 tstart[256], gbase[256], run[256];
+    __shared__ u32 wc[NWAVES][256];
+    __shared__ u32 red[NWAVES + 1];
+    const u32 t = blockIdx.x;
+    const u32 dg = threadIdx.x;
+    {
+        u64 o = (u64)dg * nt + t;
+        u32 g = off[o];
+        u32 c = off[o + 1] - g;
+        u32 tot;
+        u32 ex = block_exclusive_sum(c, red, &tot);
+        tstart[dg] = ex;
+        gbase[dg] = g;
+        run[dg] = 0;
+        #pragma unroll
+        for (int w = 0; w < NWAVES; w++) wc[w][dg] = 0;
+    }
+    __syncthreads();
+    const u64 base = (u64)t * RS_TILE;
+    const int w = wave_id();
+    #pragma unroll 1
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        bool v = i < n;
+        u64 key = v ? ki[i] : 0ull;
+        u32 val = v ? vi[i] : 0u;
+        u32 dk = (u32)(key >> shift) & 255u;
+        u64 peers = match_key8(v, dk);
+        u32 rk = (u32)__popcll(peers & lanemask_lt());
+        if (v && rk == 0) wc[w][dk] = (u32)__popcll(peers);
+        __syncthreads();
+        if (v) {
+            u32 lp = tstart[dk] + run[dk] + rk;
+            for (int w2 = 0; w2 < w; w2++) lp += wc[w2][dk];
+            sk[lp] = key;
+            sv[lp] = val;
+        }
+        __syncthreads();
+        {
+            u32 s = 0;
+            #pragma unroll
+            for (int w2 = 0; w2 < NWAVES; w2++) { s += wc[w2][dg]; wc[w2][dg] = 0; }
+            run[dg] += s;
+        }
+        __syncthreads();
+    }
+    u64 cnt = n - base < RS_TILE ? n - base : RS_TILE;
+    for (u32 j = threadIdx.x; j < cnt; j += BLOCK) {
+        u64 key = sk[j];
+        u32 dk = (u32)(key >> shift) & 255u;
+        u64 g = (u64)gbase[dk] + (j - tstart[dk]);
+        ko[g] = key;
+        vo[g] = sv[j];
+    }
+}
+
 inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int bits) {
     bool swapped = false;
+    if (n == 0) return false;
+    Arena::Mark mk = d.a->mark();
+    const u32 nt = ceil_div(n, RS_TILE);
+    u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
+    const u64 total = (u64)256 * nt;
     for (int shift = 0; shift < bits; shift += 8) {
         const u64 *ki = swapped ? kt : k;
         const u32 *vi = swapped ? vt : v;
         u64 *ko = swapped ? k : kt;
         u32 *vo = swapped ? v : vt;
-        const int sh = shift;
-        multisplit(
-            d, n,
-            [=] __device__(u64 i, u32 *key) {
-                *key = (u32)(ki[i] >> sh) & 255u;
-                return true;
-            },
-            [=] __device__(u64 i, u32, u32 gpos, u32) {
-                ko[gpos] = ki[i];
-                vo[gpos] = vi[i];
-            },
-            nullptr, "radix", 8.0 * n, 24.0 * n);
+        int pc = d.pbegin("radix_count", 8.0 * n);
+        k_rs_count<<<nt, BLOCK, 0, d.s>>>(ki, n, shift, hist, nt);
+        PSK_LAUNCH_CHECK();
+        d.pend(pc);
+        scan_exclusive(
+            d, total, [=] __device__(u64 i) { return hist[i]; },
+            [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
+        int ps = d.pbegin("radix_scatter", 24.0 * n);
+        k_rs_scatter<<<nt, BLOCK, 0, d.s>>>(ki, vi, ko, vo, n, shift, hist, nt);
+        PSK_LAUNCH_CHECK();
+        d.pend(ps);
         swapped = !swapped;
     }
+    d.a->reset(mk);
     return swapped;
 }
 
