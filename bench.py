@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override text length (testing only)")
     ap.add_argument("--cpu-sample", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -162,6 +164,14 @@ def main():
     t1 = time.perf_counter()
     kstats = P.kernel_stats(ctx)
     P.kernel_profile(ctx, False)
+    if args.calibrate_copy:
+        # a plain copy of exactly 2^30 bytes (2^28 float32): reads 1 GiB, writes 1 GiB
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     mbps = n / (ms_per_step / 1000.0) / 1e6
     rows = ctx.last_levels()
@@ -180,8 +190,15 @@ def main():
     th = time.perf_counter() - th0
     h2h_equal = bool(np.array_equal(sa_h, sa_host))
 
-    # dominant kernel roofline
+    # dominant kernel roofline; HBM traffic per launch from the committed
+    # FETCH_SIZE/WRITE_SIZE summary of the same kernel and workload, if any
     dom = max(kstats, key=lambda k: k["ms"]) if kstats else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if dom and os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if tj.get("kernel") == dom["name"] and args.config == "markov1g":
+            traffic = int(tj["traffic_bytes_per_launch"])
     roof = None
     if dom and dom["launches"]:
         avg_ms = dom["ms"] / dom["launches"]
@@ -190,7 +207,7 @@ def main():
         roof = {
             "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+            "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
             "bytes_per_launch": int(bytes_per_launch), "launches": dom["launches"],
             "share_of_device_time": round(dom["ms"] / max(sum(dev_ms), 1e-9), 4),
         }

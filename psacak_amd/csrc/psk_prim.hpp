@@ -436,14 +436,14 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
         const u32 *vi = swapped ? vt : v;
         u64 *ko = swapped ? k : kt;
         u32 *vo = swapped ? v : vt;
-        int pc = d.pbegin("radix_count", 8.0 * n);
+        int pc = d.pbegin("k_rs_count", 8.0 * n);
         k_rs_count<<<nt, BLOCK, 0, d.s>>>(ki, n, shift, hist, nt);
         PSK_LAUNCH_CHECK();
         d.pend(pc);
         scan_exclusive(
             d, total, [=] __device__(u64 i) { return hist[i]; },
             [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
-        int ps = d.pbegin("radix_scatter", 24.0 * n);
+        int ps = d.pbegin("k_rs_scatter", 24.0 * n);
         k_rs_scatter<<<nt, BLOCK, 0, d.s>>>(ki, vi, ko, vo, n, shift, hist, nt);
         PSK_LAUNCH_CHECK();
         d.pend(ps);
