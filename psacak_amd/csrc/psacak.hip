@@ -379,6 +379,21 @@ int psk_profile_stats(psk_ctx *ctx, char *names, uint64_t names_len, uint64_t *l
     return rc == PSK_OK ? count : rc;
 }
 
+// Texts are grouped into sub-batches of at most kBatchCap bytes (each one
+// level-0 run over their concatenation); a text above the cap runs alone.
+static const u64 kBatchCap = 1ull << 30;
+
+static void batch_groups(const uint64_t *lens, uint64_t count,
+                         std::vector<std::pair<u64, u64>> &groups) {
+    u64 i = 0;
+    while (i < count) {
+        u64 j = i, tot = 0;
+        while (j < count && (j == i || tot + lens[j] <= kBatchCap)) tot += lens[j++];
+        groups.push_back({i, j});
+        i = j;
+    }
+}
+
 int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
                uint32_t *const *sas, uint64_t count) {
     if (count && (!texts || !lens || !sas)) return PSK_EINVAL;
@@ -386,26 +401,70 @@ int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
         int rc = check8(texts[i], lens[i], sas[i], lens[i]);
         if (rc != PSK_OK) return rc;
     }
-    for (uint64_t i = 0; i < count; i++) {
-        int rc = psk_psacak_inplace(ctx, texts[i], lens[i], sas[i], lens[i]);
-        if (rc != PSK_OK) return rc;
-    }
-    return PSK_OK;
+    if (count == 0) return PSK_OK;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        std::vector<std::pair<u64, u64>> groups;
+        batch_groups(lens, count, groups);
+        Timer t(c);
+        for (auto &g : groups) {
+            std::vector<u64> off(g.second - g.first + 1, 0);
+            for (u64 i = g.first; i < g.second; i++) off[i - g.first + 1] = off[i - g.first] + lens[i];
+            const u64 N = off.back(), K = g.second - g.first;
+            if (N == 0) continue;
+            Arena::Mark mk = c->arena.mark();
+            try {
+                u8 *dt = c->arena.get<u8>(N);
+                u32 *ds = c->arena.get<u32>(N);
+                std::vector<u8> stage(N);
+                for (u64 i = 0; i < K; i++) memcpy(stage.data() + off[i], texts[g.first + i], lens[g.first + i]);
+                c->rt.d.upload(dt, stage.data(), N);
+                if (K == 1) psacak8_run(c->rt, dt, N, ds);
+                else batch8_run(c->rt, dt, off.data(), K, ds);
+                std::vector<u32> out(N);
+                PSK_HIP(hipMemcpyAsync(out.data(), ds, N * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
+                PSK_HIP(hipStreamSynchronize(c->stream));
+                for (u64 i = 0; i < K; i++)
+                    memcpy(sas[g.first + i], out.data() + off[i], lens[g.first + i] * sizeof(u32));
+            } catch (...) {
+                c->arena.reset(mk);
+                throw;
+            }
+            c->arena.reset(mk);
+        }
+        t.stop();
+    });
 }
 
 int psk_batch8_dev(psk_ctx *ctx, const uint8_t *d_texts, const uint64_t *offsets, uint64_t count,
                    uint32_t *d_sas, void *stream) {
     if (count && (!d_texts || !offsets || !d_sas)) return PSK_EINVAL;
+    std::vector<uint64_t> lens(count);
     for (uint64_t i = 0; i < count; i++) {
         if (offsets[i + 1] < offsets[i]) return PSK_EINVAL;
-        if (offsets[i + 1] - offsets[i] >= 0xFFFFFFFFull) return PSK_ETOOLONG;
+        lens[i] = offsets[i + 1] - offsets[i];
+        if (lens[i] >= 0xFFFFFFFFull) return PSK_ETOOLONG;
     }
-    for (uint64_t i = 0; i < count; i++) {
-        uint64_t o = offsets[i], n = offsets[i + 1] - o;
-        int rc = psk_psacak8_dev(ctx, d_texts + o, n, d_sas + o, stream);
-        if (rc != PSK_OK) return rc;
-    }
-    return PSK_OK;
+    if (count == 0) return PSK_OK;
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] {
+            std::vector<std::pair<u64, u64>> groups;
+            batch_groups(lens.data(), count, groups);
+            Timer t(c);
+            for (auto &g : groups) {
+                const u64 base = offsets[g.first], K = g.second - g.first;
+                std::vector<u64> off(K + 1);
+                for (u64 i = 0; i <= K; i++) off[i] = offsets[g.first + i] - base;
+                if (off[K] == 0) continue;
+                if (K == 1) psacak8_run(c->rt, d_texts + base, off[K], d_sas + base);
+                else batch8_run(c->rt, d_texts + base, off.data(), K, d_sas + base);
+            }
+            t.stop();
+        });
+    });
 }
 
 int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32_t *d_sa,

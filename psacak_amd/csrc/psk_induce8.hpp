@@ -42,15 +42,23 @@ struct QPos {
 
 constexpr u32 CHAIN_SMALL = 8192;
 
+// Predecessor of suffix x: none for x = 0 and, in a batch of concatenated
+// texts, none at a text's first position (start bit vector SB).
+struct Pred {
+    const u8 *T;
+    const u32 *SB;
+    __device__ __forceinline__ bool has(u32 x) const { return x > 0 && !(SB && getbit(SB, x)); }
+    __device__ __forceinline__ u32 byte(u32 x) const { return has(x) ? (u32)T[x - 1] : 0u; }
+};
+
 // Every queue entry x gets its predecessor byte cached in dig[j] (j = read
 // index), gathered once: the chain step needs "T[x-1] == c" and the
-// multi-split that follows needs T[x-1] as its digit.  Suffix 0 has no
-// predecessor; consumers check x > 0 themselves.
-__device__ __forceinline__ u32 pred_byte(const u8 *T, u32 x) { return x ? (u32)T[x - 1] : 0u; }
+// multi-split that follows needs T[x-1] as its digit.  A suffix without a
+// predecessor caches 0; consumers test Pred::has themselves.
 
 // One workgroup finishes a queue: generations until the queue is full (limit)
 // or a generation is empty.  A lone chain is followed by one thread.
-__global__ void k_chain_finish(const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c, u64 gs, u64 ge,
+__global__ void k_chain_finish(Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c, u64 gs, u64 ge,
                                u64 limit, u32 *err) {
     __shared__ u32 lds[NWAVES + 1];
     while (gs < ge) {
@@ -61,9 +69,9 @@ __global__ void k_chain_finish(const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c, u6
                 u64 j = gs;
                 u32 x = SA[qp(j)];
                 for (;;) {
-                    u32 cp = pred_byte(T, x);
+                    u32 cp = pr.byte(x);
                     dig[j] = (u8)cp;
-                    if (!(x > 0 && cp == c)) break;
+                    if (!(pr.has(x) && cp == c)) break;
                     if (j + 1 >= limit) { atomicOr(err, 1u); break; }
                     SA[qp(j + 1)] = x - 1;
                     x--;
@@ -80,9 +88,9 @@ __global__ void k_chain_finish(const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c, u6
             bool v = false;
             if (j < ge) {
                 x = SA[qp(j)];
-                u32 cp = pred_byte(T, x);
+                u32 cp = pr.byte(x);
                 dig[j] = (u8)cp;
-                v = x > 0 && cp == c;
+                v = pr.has(x) && cp == c;
             }
             u32 tot;
             u32 ex = block_exclusive_sum((u32)v, lds, &tot);
@@ -101,26 +109,26 @@ __global__ void k_chain_finish(const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c, u6
 // Fill one bucket's queue: positions qp(0..limit); generation 0 is
 // [0, *d_start).  Caches every entry's predecessor byte in dig[0..limit).
 // d_tmp: one device u32.
-inline void chain_queue8(Dev &d, const u8 *T, u32 *SA, u8 *dig, QPos qp, u32 c,
+inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
                          const u32 *d_start, u64 limit, u32 *d_tmp) {
     u64 gs = 0, ge = d.read_u32(d_start);
     u32 *err = d.err;
     while (ge > gs) {
         u64 cnt = ge - gs;
         if (cnt <= CHAIN_SMALL) {
-            k_chain_finish<<<1, BLOCK, 0, d.s>>>(T, SA, dig, qp, c, gs, ge, limit, err);
+            k_chain_finish<<<1, BLOCK, 0, d.s>>>(pr, SA, dig, qp, c, gs, ge, limit, err);
             PSK_LAUNCH_CHECK();
             return;
         }
         const u64 g0 = gs, e0 = ge, lim = limit;
         // gather the predecessor bytes of this generation (the only gather)
-        int pr = d.pbegin("chain8_gather", 9.0 * cnt);
-        for_each(d, cnt, [=] __device__(u64 i) { dig[g0 + i] = (u8)pred_byte(T, SA[qp(g0 + i)]); });
-        d.pend(pr);
+        int prec = d.pbegin("chain8_gather", 9.0 * cnt);
+        for_each(d, cnt, [=] __device__(u64 i) { dig[g0 + i] = (u8)pr.byte(SA[qp(g0 + i)]); });
+        d.pend(prec);
         if (ge >= limit) return;  // queue full: this was the last generation
         scan_exclusive(
             d, cnt,
-            [=] __device__(u64 i) { return (u32)(dig[g0 + i] == c && SA[qp(g0 + i)] > 0); },
+            [=] __device__(u64 i) { return (u32)(dig[g0 + i] == c && pr.has(SA[qp(g0 + i)])); },
             [=] __device__(u64 i, u32 off, u32 v) {
                 if (v) {
                     if (e0 + off < lim) SA[qp(e0 + off)] = SA[qp(g0 + i)] - 1;
@@ -141,25 +149,41 @@ struct Induce8Tables {
 
 // Full induce (L-scan then S-scan) over SA for a byte text, from the LMS
 // entries already placed at the bucket tails.  dig: scratch of >= n bytes.
+// Batches (SB != nullptr): K texts concatenated, ends[0..K) their last
+// positions; the L-scan starts from every text's sentinel, in text order.
 inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Induce8Tables tb,
-                    u8 *dig) {
+                    u8 *dig, const u32 *SB = nullptr, const u32 *ends = nullptr, u64 K = 1) {
+    const Pred pr{T, SB};
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
     u32 *err = d.err;
     const u64 nn = n;
     // ---------------- L-scan ----------------
     d.zero(fill, 256 * sizeof(u32));
-    {
+    if (!ends) {
         for_each(d, 1, [=] __device__(u64) {
             u32 c = T[nn - 1];
             SA[head[c]] = (u32)(nn - 1);  // the sentinel induces n-1 first
             fill[c] = 1;
         });
+    } else {
+        // text i's sentinel induces its last position; sentinels read in text order
+        multisplit(
+            d, K,
+            [=] __device__(u64 i, u32 *key) {
+                *key = T[ends[i]];
+                return true;
+            },
+            [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
+                SA[head[key] + (gpos - dstart)] = ends[i];
+            },
+            counts);
+        for_each(d, 256, [=] __device__(u64 k) { fill[k] = counts[k]; });
     }
     for (int c = 0; c < 256; c++) {
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], nlms = B.nLMS[c];
         const u32 cc = (u32)c;
-        if (nl) chain_queue8(d, T, SA, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
+        if (nl) chain_queue8(d, pr, SA, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
         const u64 R = nl + nlms;
         if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
@@ -168,8 +192,8 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
                 // the queue part has its predecessor bytes cached
-                u32 cp = j < nl ? (u32)dig[j] : pred_byte(T, x);
-                u32 k = (x > 0 && cp > cc) ? cp : 0u;
+                u32 cp = j < nl ? (u32)dig[j] : pr.byte(x);
+                u32 k = (pr.has(x) && cp > cc) ? cp : 0u;
                 dig[j] = (u8)k;  // 0 = not induced (valid digits are > c >= 0)
                 *key = k;
                 return k != 0;
@@ -198,7 +222,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
         const u32 cc = (u32)c;
-        if (ns) chain_queue8(d, T, SA, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
+        if (ns) chain_queue8(d, pr, SA, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };
@@ -206,8 +230,8 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
             d, R,
             [=] __device__(u64 j, u32 *key) {
                 u32 x = SA[pos(j)];
-                u32 cp = j < ns ? (u32)dig[j] : pred_byte(T, x);
-                u32 k = (x > 0 && cp < cc) ? cp : 255u;
+                u32 cp = j < ns ? (u32)dig[j] : pr.byte(x);
+                u32 k = (pr.has(x) && cp < cc) ? cp : 255u;
                 dig[j] = (u8)k;  // 255 = not induced (valid digits are < c <= 255)
                 *key = k;
                 return k != 255;

@@ -200,7 +200,8 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lmsbits,
 // Solve the reduced problem of a level: names -> SA1 (as LMS positions).
 template <class C>
 inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
-                          const u32 *R, u32 m1, u32 *SA1, int depth, LevelStat &st) {
+                          const u32 *R, u32 m1, u32 *SA1, int depth, LevelStat &st,
+                          const u32 *EB = nullptr) {
     Dev &d = rt.d;
     if (m1 == 0) return;
     Arena::Mark mk = d.a->mark();
@@ -208,7 +209,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
     u32 *tails = d.a->get<u32>(m1);
     u32 *diff = d.a->get<u32>(m1);
     u32 *dk = d.a->get<u32>(1);
-    u32 k = name_lms(d, T, lms_sorted, lms_text, R, m1, diff, reduced, tails, dk);
+    u32 k = name_lms(d, T, lms_sorted, lms_text, R, m1, diff, reduced, tails, dk, EB);
     st.k = k;
     rt.levels[depth] = st;
     if (k < m1) {

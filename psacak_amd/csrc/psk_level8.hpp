@@ -6,13 +6,27 @@
 
 namespace psk {
 
-// SA of the byte text T[0..n) into SA (device buffers).
-inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA) {
+// A batch of texts concatenated in one array: their last positions (ends,
+// in text order) and bit vectors of last and first positions.
+struct BatchInfo {
+    const u32 *ends;
+    u64 K;
+    const u32 *endbits, *startbits;
+};
+
+// SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
+// batch of texts, each with its own sentinel, and SA receives the SA of the
+// concatenation in which every text's suffixes are in their own order.
+inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInfo *bi = nullptr) {
     Dev &d = rt.d;
     rt.levels.clear();
     rt.max_rounds = 0;
     if (n == 0) return;
-    if (n <= 3) {  // psacak8.rs:23-26
+    const u32 *EB = bi ? bi->endbits : nullptr;
+    const u32 *SB = bi ? bi->startbits : nullptr;
+    const u32 *ENDS = bi ? bi->ends : nullptr;
+    const u64 K = bi ? bi->K : 1;
+    if (n <= 3 && !bi) {  // psacak8.rs:23-26
         k_naive_small<u8><<<1, 1, 0, d.s>>>(T, (u32)n, SA);
         PSK_LAUNCH_CHECK();
         return;
@@ -21,13 +35,13 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA) {
     const u64 nwords = (n + 31) / 32;
     u32 *sbits = d.a->get<u32>(nwords + 1);
     u32 *lmsbits = d.a->get<u32>(nwords + 1);
-    compute_types(d, T, n, sbits);
+    compute_types(d, T, n, sbits, EB);
 
     // bucket statistics (Buckets::new psacak8.rs:355-374)
     u32 *stat = d.a->get<u32>(3 * 256 + 1);
     d.zero(stat, (3 * 256 + 1) * sizeof(u32));
     k_lms_hist8<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(T, n, sbits, lmsbits, nwords, stat,
-                                                          stat + 256, stat + 512, stat + 768);
+                                                          stat + 256, stat + 512, stat + 768, SB);
     PSK_LAUNCH_CHECK();
     Buckets8 B;
     u32 host_stat[3 * 256 + 1];
@@ -87,7 +101,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA) {
             SA[lmsbase[key] + (gpos - dstart)] = lms_text[i];
         },
         nullptr);
-    induce8(d, T, n, SA, B, tb, dig);
+    induce8(d, T, n, SA, B, tb, dig, SB, ENDS, K);
 
     // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
     u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
@@ -100,14 +114,68 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA) {
 
     // ---- reduced problem (psacak8.rs:36-48) ---------------------------------
     u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
-    solve_reduced(rt, T, lms_sorted, lms_text, R, m1, SA1, 0, st);
+    solve_reduced(rt, T, lms_sorted, lms_text, R, m1, SA1, 0, st, EB);
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
     for_each(d, m1, [=] __device__(u64 j) {
         u32 p = SA1[j];
         SA[j + lmsoff[T[p]]] = p;
     });
-    induce8(d, T, n, SA, B, tb, dig);
+    induce8(d, T, n, SA, B, tb, dig, SB, ENDS, K);
+    d.a->reset(mk);
+}
+
+// SAs of K texts stored back to back in T: text i is T[off[i] .. off[i+1])
+// (off: host array, off[0] = 0, off[K] = N <= 2^32-2).  SA[off[i] + r] is
+// the r-th smallest suffix of text i, as a position inside text i.  All texts
+// share every kernel launch: one level-0 run over the concatenation in which
+// each text has its own sentinel, then a stable partition by text id.
+inline void batch8_run(Runtime &rt, const u8 *T, const u64 *off, u64 K, u32 *SA) {
+    Dev &d = rt.d;
+    const u64 N = off[K];
+    if (N == 0) return;
+    Arena::Mark mk = d.a->mark();
+    const u64 nwords = (N + 31) / 32;
+    u32 *EB = d.a->get<u32>(nwords + 1);
+    u32 *SB = d.a->get<u32>(nwords + 1);
+    d.zero(EB, (nwords + 1) * sizeof(u32));
+    d.zero(SB, (nwords + 1) * sizeof(u32));
+    std::vector<u32> hoff(K + 1), hends;
+    hends.reserve(K);
+    for (u64 i = 0; i <= K; i++) hoff[i] = (u32)off[i];
+    for (u64 i = 0; i < K; i++)
+        if (off[i + 1] > off[i]) hends.push_back((u32)(off[i + 1] - 1));
+    u32 *doff = d.a->get<u32>(K + 1);
+    u32 *dends = d.a->get<u32>(hends.size() + 1);
+    d.upload(doff, hoff.data(), (K + 1) * sizeof(u32));
+    d.upload(dends, hends.data(), hends.size() * sizeof(u32));
+    const u64 KK = K;
+    for_each(d, K, [=] __device__(u64 i) {
+        u32 a = doff[i], b = doff[i + 1];
+        if (b > a) {
+            atomicOr(&SB[a >> 5], 1u << (a & 31));
+            atomicOr(&EB[(b - 1) >> 5], 1u << ((b - 1) & 31));
+        }
+    });
+    BatchInfo bi{dends, hends.size(), EB, SB};
+    psacak8_run(rt, T, N, SA, &bi);
+    // stable partition of the concatenation's SA by text id
+    u64 *k0 = d.a->get<u64>(N), *k1 = d.a->get<u64>(N);
+    u32 *v0 = d.a->get<u32>(N), *v1 = d.a->get<u32>(N);
+    for_each(d, N, [=] __device__(u64 j) {
+        u32 x = SA[j];
+        u64 lo = 0, hi = KK;  // last i with doff[i] <= x, among non-empty texts
+        while (hi - lo > 1) {
+            u64 mid = (lo + hi) / 2;
+            if (doff[mid] <= x) lo = mid;
+            else hi = mid;
+        }
+        while (lo + 1 < KK && doff[lo + 1] <= x) lo++;  // skip empty texts ending at x
+        k0[j] = lo;
+        v0[j] = x - doff[lo];
+    });
+    bool sw = radix_sort_kv(d, N, k0, v0, k1, v1, bit_width(K > 1 ? K - 1 : 0));
+    PSK_HIP(hipMemcpyAsync(SA, sw ? v1 : v0, N * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
     d.a->reset(mk);
 }
 

@@ -26,18 +26,34 @@ __device__ __forceinline__ bool lms_substr_equal(const C *T, u32 a, u32 b, u32 l
     return true;
 }
 
+// Any set bit of bits in positions [p, q)?
+__device__ __forceinline__ bool range_has_bit(const u32 *bits, u32 p, u32 q) {
+    for (u32 w = p >> 5; w <= ((q - 1) >> 5); w++) {
+        u32 x = bits[w];
+        if (w == (p >> 5)) x &= 0xFFFFFFFFu << (p & 31);
+        if (w == ((q - 1) >> 5)) x &= 0xFFFFFFFFu >> (31 - ((q - 1) & 31));
+        if (x) return true;
+    }
+    return false;
+}
+
 // Returns k (number of distinct names).  reduced[r] = name of the r-th LMS
 // position in text order; tails[c] = number of LMS substrings with name <= c.
+// Batches: EB marks each text's last position; an LMS substring that would
+// run past it reaches that text's sentinel and equals nothing.
 template <class C>
 inline u32 name_lms(Dev &d, const C *T, const u32 *lms_sorted, const u32 *lms_text,
-                    const u32 *R, u32 m1, u32 *diff, u32 *reduced, u32 *tails, u32 *d_k) {
+                    const u32 *R, u32 m1, u32 *diff, u32 *reduced, u32 *tails, u32 *d_k,
+                    const u32 *EB = nullptr) {
     if (m1 == 0) return 0;
     for_each(d, m1, [=] __device__(u64 j) {
         if (j == 0) { diff[0] = 1; return; }
         u32 a = lms_sorted[j - 1], b = lms_sorted[j];
         u32 ra = R[a >> 1], rb = R[b >> 1];
         if (ra + 1 == m1 || rb + 1 == m1) { diff[j] = 1; return; }  // sentinel substring
-        u32 la = lms_text[ra + 1] - a + 1, lb = lms_text[rb + 1] - b + 1;
+        u32 na = lms_text[ra + 1], nb = lms_text[rb + 1];
+        if (EB && (range_has_bit(EB, a, na) || range_has_bit(EB, b, nb))) { diff[j] = 1; return; }
+        u32 la = na - a + 1, lb = nb - b + 1;
         diff[j] = (la != lb || !lms_substr_equal(T, a, b, la)) ? 1u : 0u;
     });
     scan_exclusive(

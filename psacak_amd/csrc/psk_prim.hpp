@@ -42,6 +42,11 @@ struct Dev {
         PSK_HIP(hipMemcpyAsync(d, pinned + 2048, bytes, hipMemcpyHostToDevice, s));
         PSK_HIP(hipStreamSynchronize(s));
     }
+    // arbitrary-size host -> device copy (pageable source), synchronous
+    void upload(void *d, const void *host, u64 bytes) {
+        if (bytes) PSK_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s));
+        PSK_HIP(hipStreamSynchronize(s));
+    }
     void zero(void *d, u64 bytes) {
         if (bytes) PSK_HIP(hipMemsetAsync(d, 0, bytes, s));
     }

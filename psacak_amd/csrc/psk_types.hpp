@@ -20,8 +20,20 @@ namespace psk {
 constexpr int TY_WPT = 4;
 constexpr u64 TY_POS_PER_BLOCK = (u64)BLOCK * TY_WPT * 32;
 
+// A batch of texts concatenated into one array carries an end bit vector
+// (bit set at each text's last position): a text end is always L-type, as if
+// followed by its own sentinel, so no carry crosses a text boundary.
+__device__ __forceinline__ void apply_ends(const u32 *endbits, u64 p0, u64 n, u32 &D, u32 &V) {
+    if (endbits && p0 < n) {
+        u32 e = endbits[p0 >> 5];
+        D |= e;
+        V &= ~e;
+    }
+}
+
 template <class C>
-__device__ __forceinline__ void type_masks(const C *t, u64 n, u64 p0, u32 &D, u32 &V) {
+__device__ __forceinline__ void type_masks(const C *t, u64 n, u64 p0, u32 &D, u32 &V,
+                                           const u32 *endbits) {
     D = 0;
     V = 0;
     C a = (p0 < n) ? t[p0] : (C)0;
@@ -39,12 +51,14 @@ __device__ __forceinline__ void type_masks(const C *t, u64 n, u64 p0, u32 &D, u3
             D |= 1u << b;  // p == n-1 is L-type; p >= n is padding (L)
         }
     }
+    apply_ends(endbits, p0, n, D, V);
 }
 
 // Byte texts: 32 bytes (+1) per word with dword loads when the text is
 // 4-byte aligned and the word is not the last one.
 template <>
-__device__ __forceinline__ void type_masks<u8>(const u8 *t, u64 n, u64 p0, u32 &D, u32 &V) {
+__device__ __forceinline__ void type_masks<u8>(const u8 *t, u64 n, u64 p0, u32 &D, u32 &V,
+                                               const u32 *endbits) {
     if (p0 + 33 <= n && (((uintptr_t)t) & 3u) == 0) {
         const u32 *q = (const u32 *)(t + p0);
         u32 w[9];
@@ -60,6 +74,7 @@ __device__ __forceinline__ void type_masks<u8>(const u8 *t, u64 n, u64 p0, u32 &
             D |= (u32)(a != c) << b;
             V |= (u32)(a < c) << b;
         }
+        apply_ends(endbits, p0, n, D, V);
         return;
     }
     D = 0;
@@ -76,6 +91,7 @@ __device__ __forceinline__ void type_masks<u8>(const u8 *t, u64 n, u64 p0, u32 &
             D |= 1u << b;
         }
     }
+    apply_ends(endbits, p0, n, D, V);
 }
 
 // Type bits of one word given the type of the position right after it.
@@ -112,14 +128,14 @@ __device__ __forceinline__ u32 carry_from_right(bool has, u32 first, u32 block_c
 }
 
 template <class C>
-__global__ void k_type_agg(const C *t, u64 n, u8 *blk_has, u8 *blk_first) {
+__global__ void k_type_agg(const C *t, u64 n, u8 *blk_has, u8 *blk_first, const u32 *endbits) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
     u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
     bool has = false;
     u32 first = 0;
     for (int k = 0; k < TY_WPT; k++) {
         u32 D, V;
-        type_masks(t, n, (w0 + k) * 32, D, V);
+        type_masks(t, n, (w0 + k) * 32, D, V, endbits);
         if (!has && D) {
             has = true;
             first = (V >> (__ffs(D) - 1)) & 1u;
@@ -185,7 +201,8 @@ __global__ void k_type_carry(const u8 *blk_has, const u8 *blk_first, u32 nb, u8 
 }
 
 template <class C>
-__global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits, u64 nwords) {
+__global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits, u64 nwords,
+                             const u32 *endbits) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
     u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
     u32 D[TY_WPT], V[TY_WPT];
@@ -193,7 +210,7 @@ __global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits,
     u32 first = 0;
     #pragma unroll
     for (int k = 0; k < TY_WPT; k++) {
-        type_masks(t, n, (w0 + k) * 32, D[k], V[k]);
+        type_masks(t, n, (w0 + k) * 32, D[k], V[k], endbits);
         if (!has && D[k]) {
             has = true;
             first = (V[k] >> (__ffs(D[k]) - 1)) & 1u;
@@ -210,7 +227,7 @@ __global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits,
 
 // S-type bit vector of t[0..n) into sbits[0 .. ceil(n/32)).
 template <class C>
-inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits) {
+inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endbits = nullptr) {
     if (n == 0) return;
     u64 nwords = (n + 31) / 32;
     u32 nb = ceil_div(nwords, (u64)BLOCK * TY_WPT);
@@ -218,11 +235,11 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits) {
     u8 *bh = d.a->get<u8>(nb);
     u8 *bf = d.a->get<u8>(nb);
     u8 *bc = d.a->get<u8>(nb);
-    k_type_agg<C><<<nb, BLOCK, 0, d.s>>>(t, n, bh, bf);
+    k_type_agg<C><<<nb, BLOCK, 0, d.s>>>(t, n, bh, bf, endbits);
     PSK_LAUNCH_CHECK();
     k_type_carry<<<1, BLOCK, 0, d.s>>>(bh, bf, nb, bc);
     PSK_LAUNCH_CHECK();
-    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords);
+    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords, endbits);
     PSK_LAUNCH_CHECK();
     d.a->reset(mk);
 }
@@ -239,8 +256,10 @@ __device__ __forceinline__ u32 lms_word(const u32 *sbits, u64 w) {
 // cnt[c] occurrences, nL[c] L-type occurrences, nLMS[c] LMS occurrences;
 // *m1 = number of LMS positions.  (psacak8.rs:355-374 Buckets::new)
 constexpr u64 H_F = 21;  // bits per packed LDS counter field
+// startbits (batches): a text's first position has no predecessor in its
+// text, so it is never LMS.
 __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, u64 nwords,
-                            u32 *cnt, u32 *nL, u32 *nLMS, u32 *m1) {
+                            u32 *cnt, u32 *nL, u32 *nLMS, u32 *m1, const u32 *startbits) {
     __shared__ unsigned long long h[256];
     __shared__ u32 red[NWAVES + 1];
     h[threadIdx.x] = 0;
@@ -250,6 +269,7 @@ __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, 
     if (w < nwords) {
         u32 S = sbits[w];
         u32 lms = lms_word(sbits, w);
+        if (startbits) lms &= ~startbits[w];
         lmsbits[w] = lms;
         lcount = __popc(lms);
         u64 p0 = w * 32;

@@ -182,12 +182,42 @@ def test_device_api_and_checker(ctx):
 
 
 def test_batch(ctx):
-    note("batch8")
-    L = gen.lengths(40, 4, 20000, seed=8)
+    note("batch8 mixed sizes")
+    rng = np.random.default_rng(8)
+    L = gen.lengths(60, 4, 20000, seed=8)
     texts = [gen.iid_dna(int(n), seed=i) for i, n in enumerate(L)]
+    texts += [b"", b"a", b"ab", b"ba", b"aaa", b"mississippi", b"", bytes(range(255, 0, -1))]
+    texts += [rng.integers(0, 3, int(rng.integers(1, 50))).astype(np.uint8) for _ in range(40)]
     outs = P.batch8(texts, ctx)
-    for t, o in zip(texts, outs):
-        assert np.array_equal(o, oracle.sacak8(t))
+    for i, (t, o) in enumerate(zip(texts, outs)):
+        assert np.array_equal(o, oracle.sacak8(t)), i
+
+
+def test_batch_many_small_and_dev(ctx):
+    import torch
+    note("batch8_dev 3000 texts")
+    L = gen.lengths(3000, 1, 3000, seed=21)
+    texts = [gen.markov_dna(int(n), seed=100 + i, repeat_frac=0.0) for i, n in enumerate(L)]
+    off = np.zeros(len(texts) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(t) for t in texts])
+    cat = np.concatenate(texts)
+    dt = torch.from_numpy(cat).to("cuda:0")
+    ds = torch.zeros(len(cat), dtype=torch.int32, device="cuda:0")
+    P.batch8_dev(dt, off, ds, ctx=ctx)
+    torch.cuda.synchronize()
+    sa = ds.cpu().numpy().view(np.uint32)
+    for i, t in enumerate(texts):
+        assert np.array_equal(sa[off[i]:off[i + 1]], oracle.sacak8(t)), i
+
+
+def test_batch_matches_single_calls_on_repeats(ctx):
+    note("batch8 identical texts")
+    t = gen.markov_dna(50000, seed=5)
+    texts = [t, t[:49999], t, t[1:], t[:1000] * 50]
+    outs = P.batch8(texts, ctx)
+    for x, o in zip(texts, outs):
+        assert np.array_equal(o, P.psacak(x, ctx))
+        assert np.array_equal(o, oracle.sacak8(x))
 
 
 def test_budget_is_enforced():
