@@ -27,9 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 CONFIGS = {
-    # name: (description, generator kwargs)
     "markov1g": "plain GPU psacak8, 1 GiB order-3 Markov DNA + 10% repeats",
     "iid64m": "plain psacak8, 64 MiB i.i.d. uniform ACGT",
+    "zipf32": "psacak32 (translate + integer SACA), 256 Mi u32 Zipf(1.1) over 2^20 + 20% repeats",
+    "batch10k": "batched psacak8, 10,000 i.i.d. DNA texts, lengths log-uniform 1 KiB-4 MiB",
 }
 
 
@@ -116,6 +117,167 @@ def cpu_baseline(text, sample_bytes):
     }
 
 
+def bench_zipf32(args):
+    """psacak32 at the recursion-level shape: one call of psacak32_raw (the
+    translation to bucket-pointer form plus psacak32) on 2^28 u32 symbols."""
+    import numpy as np
+    import torch
+
+    import psacak_amd as P
+    from psacak_amd import gen
+
+    n = args.n or (1 << 28)
+    log("generating zipf32")
+    text = gen.zipf32(n, sigma=1 << 20, s=1.1, repeat_frac=0.20, seed=20260103)
+    dev = torch.device("cuda:0")
+    d_text = torch.from_numpy(text.view(np.int32)).to(dev)
+    d_sa = torch.empty(n, dtype=torch.int32, device=dev)
+    ctx = P.Context()
+    for _ in range(args.warmup):
+        P.psacak32_raw_dev(d_text, d_sa, n, ctx=ctx)
+    torch.cuda.synchronize()
+    P.kernel_profile(ctx, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.psacak32_raw_dev(d_text, d_sa, n, ctx=ctx)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kstats = P.kernel_stats(ctx)
+    P.kernel_profile(ctx, False)
+    ms = (t1 - t0) * 1000.0 / args.steps
+    log("parity sample")
+    sa = d_sa.cpu().numpy().view(np.uint32)
+    # permutation + sampled adjacent pairs on the host (u32 symbols)
+    perm_ok = bool(np.array_equal(np.bincount(sa, minlength=n), np.ones(n, dtype=np.int64)))
+    rng = np.random.default_rng(3)
+    bad = 0
+    for i in rng.integers(1, n, 2000):
+        a, b = int(sa[i - 1]), int(sa[i])
+        k = 0
+        while a + k < n and b + k < n and text[a + k] == text[b + k]:
+            k += 1
+        ok = (a + k == n) or (b + k < n and text[a + k] < text[b + k])
+        bad += not ok
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        m = min(n, 1 << 23)
+        t2 = time.perf_counter()
+        ref = oracle.sacak32_raw(text[:m])
+        dt = time.perf_counter() - t2
+        sub_sa = P.psacak32_raw(text[:m], ctx)
+        cpu = {"value": round(m * 4 / dt / 1e6, 3), "unit": "MB/s", "cores": 1, "kind": "port",
+               "sample": f"first {m} symbols, serial C restatement (translate + psacak32), "
+                         f"{dt:.2f} s; GPU SA of the sample equal: "
+                         f"{bool(np.array_equal(ref, sub_sa))}"}
+    dom = max(kstats, key=lambda k: k["ms"]) if kstats else None
+    roof = None
+    if dom and dom["launches"]:
+        avg = dom["ms"] / dom["launches"]
+        ach = dom["bytes"] / dom["launches"] / (avg / 1000.0) / 1e9
+        roof = {"bound": "hbm", "kernel": dom["name"], "achieved": round(ach, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_ms": round(avg, 5), "launches": dom["launches"]}
+    out = {
+        "metric": "psacak32 input MB/s at 256 Mi u32 symbols (bit-exact SA)",
+        "value": round(n * 4 / (ms / 1000.0) / 1e6, 2), "unit": "MB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32->u32",
+        "data": "synthetic (seeded generator psacak_amd/csrc/gen.c)",
+        "config": {"workload": CONFIGS["zipf32"], "n": n, "msymbols_per_s": round(n / ms / 1e3, 2)},
+        "roofline": roof, "cpu_baseline": cpu,
+        "parity": {"permutation": perm_ok, "sampled_pairs": 2000, "bad_pairs": int(bad)},
+        "levels": [[int(x) for x in r] for r in ctx.last_levels()],
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_batch(args):
+    """Batched psacak8 over 10,000 uneven texts vs one plain call per text."""
+    import numpy as np
+    import torch
+
+    import psacak_amd as P
+    from psacak_amd import gen
+
+    count = args.n or 10000
+    lens = gen.lengths(count, 1024, 4 << 20, seed=20260104)
+    log(f"generating {count} texts, {int(lens.sum())} bytes")
+    off = np.zeros(count + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(lens)
+    cat = np.empty(int(off[-1]), dtype=np.uint8)
+    for i in range(count):
+        cat[off[i]:off[i + 1]] = gen.iid_dna(int(lens[i]), seed=20260104 * 1000003 + i)
+    total = int(off[-1])
+    dev = torch.device("cuda:0")
+    d_text = torch.from_numpy(cat).to(dev)
+    d_sa = torch.empty(total, dtype=torch.int32, device=dev)
+    ctx = P.Context()
+    log("warmup")
+    for _ in range(args.warmup):
+        P.batch8_dev(d_text, off[:101], d_sa, ctx=ctx)
+    torch.cuda.synchronize()
+    log(f"timing batched x{args.steps}")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.batch8_dev(d_text, off, d_sa, ctx=ctx)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / args.steps
+    # per-text plain calls on a bounded sample of the same texts
+    sample = min(count, args.per_text_sample)
+    d_sa2 = torch.empty(int(off[sample]), dtype=torch.int32, device=dev)
+    log(f"per-text calls on the first {sample} texts")
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    base = d_text.data_ptr()
+    base2 = d_sa2.data_ptr()
+    for i in range(sample):
+        P.psacak8_dev(base + int(off[i]), base2 + 4 * int(off[i]), int(lens[i]), ctx=ctx)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    # batched on the same sample, for a like-for-like ratio
+    t4 = time.perf_counter()
+    P.batch8_dev(d_text, off[: sample + 1], d_sa2, ctx=ctx)
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    log("parity")
+    sa = d_sa.cpu().numpy().view(np.uint32)
+    sa2 = d_sa2.cpu().numpy().view(np.uint32)
+    same_as_plain = bool(np.array_equal(sa[: int(off[sample])], sa2))
+    # device check of every text (permutation + adjacent-pair condition)
+    ok_all = True
+    for i in range(count):
+        a, b = int(off[i]), int(off[i + 1])
+        if not P.check8_dev(base + a, d_sa.data_ptr() + 4 * a, b - a, ctx=ctx):
+            ok_all = False
+            break
+    import oracle
+    small = [i for i in range(count) if lens[i] <= 20000][:50]
+    oracle_ok = all(np.array_equal(sa[off[i]:off[i + 1]], oracle.sacak8(cat[off[i]:off[i + 1]]))
+                    for i in small)
+    gbps = total / (ms / 1000.0) / 1e9
+    pertext = int(off[sample]) / (t3 - t2) / 1e9
+    batched_sample = int(off[sample]) / (t5 - t4) / 1e9
+    out = {
+        "metric": "batched GB/s vs per-text (10k uneven DNA texts, bit-exact SAs)",
+        "value": round(gbps, 4), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8->u32",
+        "data": "synthetic (seeded generator psacak_amd/csrc/gen.c)",
+        "config": {"workload": CONFIGS["batch10k"], "texts": count, "bytes": total},
+        "per_text": {"sample_texts": sample, "sample_bytes": int(off[sample]),
+                     "per_text_GBps": round(pertext, 4), "batched_GBps_same_sample": round(batched_sample, 4),
+                     "speedup_same_sample": round(batched_sample / pertext, 2),
+                     "speedup_full_vs_per_text": round(gbps / pertext, 2)},
+        "parity": {"device_check_all_texts": ok_all, "batched_equals_plain_on_sample": same_as_plain,
+                   "oracle_equal_small_texts": bool(oracle_ok), "oracle_checked": len(small)},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +287,7 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override text length (testing only)")
     ap.add_argument("--cpu-sample", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-text-sample", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -132,6 +295,10 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.config == "zipf32":
+        return bench_zipf32(args)
+    if args.config == "batch10k":
+        return bench_batch(args)
 
     import numpy as np
     import torch
