@@ -31,6 +31,8 @@ CONFIGS = {
     "iid64m": "plain psacak8, 64 MiB i.i.d. uniform ACGT",
     "zipf32": "psacak32 (translate + integer SACA), 256 Mi u32 Zipf(1.1) over 2^20 + 20% repeats",
     "batch10k": "batched psacak8, 10,000 i.i.d. DNA texts, lengths log-uniform 1 KiB-4 MiB",
+    "served": "served psacak8: Poisson 200 req/s, i.i.d. DNA texts log-uniform 10 KiB-64 MiB, "
+              "1% 1-2 GiB, continuous batching under an HBM budget",
 }
 
 
@@ -278,6 +280,81 @@ def bench_batch(args):
     ctx.close()
 
 
+def bench_served(args):
+    """A Poisson request stream through psk_server (host buffers in and out)."""
+    import threading
+
+    import numpy as np
+
+    import oracle
+    import psacak_amd as P
+    from psacak_amd import gen
+
+    count = args.n or 400
+    gaps, lens = gen.requests(count, mean_ms=5.0, lo=10 << 10, hi=64 << 20, big_p=0.01,
+                              big_lo=1 << 30, big_hi=2 << 30, seed=20260105)
+    log(f"generating {count} requests, {int(lens.sum())} bytes ({int((lens >= 1 << 30).sum())} large)")
+    texts = [gen.iid_dna(int(n), seed=20260105 * 1000003 + i) for i, n in enumerate(lens)]
+    budget = args.budget_gb << 30
+    with P.Server(hbm_budget=budget, workers=args.workers) as srv:
+        # warm the workers' contexts (device init, first allocations)
+        for w in range(args.workers):
+            srv.wait(srv.submit(texts[0][: 1 << 16]))
+        ids = [0] * count
+        t_sub = [0.0] * count
+
+        def producer():
+            t_next = time.perf_counter()
+            for i in range(count):
+                t_next += gaps[i] / 1000.0
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                t_sub[i] = time.perf_counter()
+                ids[i] = srv.submit(texts[i])
+
+        log("serving")
+        t0 = time.perf_counter()
+        th = threading.Thread(target=producer)
+        th.start()
+        lat, results = [], {}
+        check = set(range(0, count, max(1, count // 25)))
+        for i in range(count):
+            while ids[i] == 0 and th.is_alive():
+                time.sleep(0.0005)
+            sa, ms = srv.wait(ids[i])
+            lat.append(ms)
+            if i in check and lens[i] <= (64 << 20):
+                results[i] = sa
+            if (i + 1) % 50 == 0:
+                log(f"{i + 1}/{count} done")
+        th.join()
+        t1 = time.perf_counter()
+        peak = srv.peak_bytes()
+    lat = np.array(lat)
+    total = int(lens.sum())
+    ok = all(oracle.check_sa8(texts[i], sa) == 0 for i, sa in results.items())
+    out = {
+        "metric": "served p50/p99 latency and throughput (Poisson DNA requests, bit-exact SAs)",
+        "value": round(total / (t1 - t0) / 1e9, 4), "unit": "GB/s", "n_gpus": 1, "steps": 1,
+        "warmup": args.workers, "ms_per_step": round((t1 - t0) * 1000, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8->u32",
+        "data": "synthetic (seeded generator psacak_amd/csrc/gen.c)",
+        "config": {"workload": CONFIGS["served"], "requests": count, "bytes": total,
+                   "large_requests": int((lens >= 1 << 30).sum()), "workers": args.workers,
+                   "hbm_budget_GB": args.budget_gb},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p90": round(float(np.percentile(lat, 90)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "offered_GBps": round(total / (float(np.sum(gaps)) / 1000.0) / 1e9, 3),
+        "sustained_GBps": round(total / (t1 - t0) / 1e9, 3),
+        "peak_hbm_GB": round(peak / 1e9, 2),
+        "parity": {"checked_requests": len(results), "all_valid": bool(ok)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +365,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-text-sample", type=int, default=300)
+    ap.add_argument("--workers", type=int, default=3)
+    ap.add_argument("--budget-gb", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (PMC calibration of FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -299,6 +378,8 @@ def main():
         return bench_zipf32(args)
     if args.config == "batch10k":
         return bench_batch(args)
+    if args.config == "served":
+        return bench_served(args)
 
     import numpy as np
     import torch

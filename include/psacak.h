@@ -37,6 +37,7 @@ extern "C" {
 #define PSK_ENOTFOUND -8 /* server: unknown request id */
 
 typedef struct psk_ctx psk_ctx;
+typedef struct psk_server psk_server;
 
 /* Human-readable text of a status code (static storage). */
 const char *psk_strerror(int status);
@@ -125,7 +126,31 @@ int psk_batch8_dev(psk_ctx *ctx, const uint8_t *d_texts, const uint64_t *offsets
  * Burkhardt-Kaerkkaeinen adjacent-pair condition).  *ok = 1 if valid. */
 int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32_t *d_sa,
                    int *ok, void *stream);
-
+/* ---- serving ------------------------------------------------------------ */
+/* No counterpart in the reference (a library call only).  A server runs
+ * max_workers worker threads, each with its own context and stream.  Requests
+ * are admitted in arrival order while the estimated device footprint of the
+ * work in flight (psk_server_footprint) stays within hbm_budget_bytes
+ * (0 = no budget); consecutive small requests are coalesced into one batch;
+ * jobs of different workers overlap on the device. */
+int psk_server_create(psk_server **out, uint64_t hbm_budget_bytes, int max_workers);
+/* Device bytes the server reserves for a job of `bytes` text bytes. */
+uint64_t psk_server_footprint(uint64_t bytes, int batched);
+/* Submit a text (copied).  *id receives the request id.  PSK_EBUDGET if the
+ * request alone exceeds the budget. */
+int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *id);
+/* *done = 1 and, when finished, the SA (n entries) is copied to sa and
+ * *latency_ms = submit-to-completion time; the request is then released.
+ * Returns the request's status (or PSK_ENOTFOUND). */
+int psk_server_poll(psk_server *s, uint64_t id, int *done, uint32_t *sa, double *latency_ms);
+/* Block until request id completes (timeout_ms < 0: no limit), then as poll. */
+int psk_server_wait(psk_server *s, uint64_t id, uint32_t *sa, double *latency_ms,
+                    double timeout_ms);
+/* Peak device bytes reserved by the server's contexts since creation. */
+uint64_t psk_server_peak_bytes(psk_server *s);
+/* Number of requests submitted but not yet finished. */
+uint64_t psk_server_pending(psk_server *s);
+void psk_server_destroy(psk_server *s);
 
 #ifdef __cplusplus
 }

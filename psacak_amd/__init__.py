@@ -25,7 +25,7 @@ __all__ = [
     "PskError", "Context", "psacak", "psacak_inplace", "psacak32", "psacak32_raw",
     "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "batch8", "batch8_dev",
     "workspace_bytes8", "workspace_bytes32", "lib_path", "version", "kernel_profile",
-    "kernel_stats",
+    "kernel_stats", "Server",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -81,6 +81,14 @@ def _load():
         "psk_batch8_dev": ([vp, vp, vp, u64, vp, vp], i32),
         "psk_check8_dev": ([vp, vp, u64, vp, ctypes.POINTER(i32), vp], i32),
         "psk_profile": ([vp, i32], i32),
+        "psk_server_create": ([ctypes.POINTER(vp), u64, i32], i32),
+        "psk_server_footprint": ([u64, i32], u64),
+        "psk_server_submit": ([vp, vp, u64, ctypes.POINTER(u64)], i32),
+        "psk_server_poll": ([vp, u64, ctypes.POINTER(i32), vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "psk_server_wait": ([vp, u64, vp, ctypes.POINTER(ctypes.c_double), ctypes.c_double], i32),
+        "psk_server_peak_bytes": ([vp], u64),
+        "psk_server_pending": ([vp], u64),
+        "psk_server_destroy": ([vp], None),
         "psk_profile_stats": ([vp, ctypes.c_char_p, u64, vp, vp, vp, i32], i32),
     }
     for name, (args, res) in sig.items():
@@ -335,3 +343,76 @@ def batch8_dev(d_texts, offsets, d_sas, ctx: Context | None = None, stream=None)
     rc = _lib.psk_batch8_dev(_ctx_handle(ctx), _dev_ptr(d_texts), off.ctypes.data,
                              len(off) - 1, _dev_ptr(d_sas), _stream_ptr(stream))
     _raise(rc, ctx)
+
+
+class Server:
+    """Continuous-batching SA server (``psk_server_*``): requests are admitted
+    in arrival order against an HBM budget, small ones are coalesced into
+    batches, and ``workers`` worker threads (one stream each) overlap their
+    jobs on the device.  Results equal single :func:`psacak` calls."""
+
+    def __init__(self, hbm_budget: int = 0, workers: int = 2):
+        h = ctypes.c_void_p()
+        rc = _lib.psk_server_create(ctypes.byref(h), hbm_budget, workers)
+        if rc != PSK_OK:
+            raise PskError(rc)
+        self._h = h
+        self._n = {}
+
+    @staticmethod
+    def footprint(nbytes: int, batched: bool = False) -> int:
+        return int(_lib.psk_server_footprint(nbytes, 1 if batched else 0))
+
+    def submit(self, text) -> int:
+        t = _as_u8(text)
+        rid = ctypes.c_uint64()
+        rc = _lib.psk_server_submit(self._h, t.ctypes.data, len(t), ctypes.byref(rid))
+        _raise(rc)
+        self._n[rid.value] = len(t)
+        return rid.value
+
+    def poll(self, rid: int):
+        """(done, sa or None, latency_ms or None); a finished request is released."""
+        done = ctypes.c_int(0)
+        lat = ctypes.c_double(0)
+        sa = np.empty(self._n.get(rid, 0), dtype=np.uint32)
+        rc = _lib.psk_server_poll(self._h, rid, ctypes.byref(done), sa.ctypes.data,
+                                  ctypes.byref(lat))
+        if not done.value and rc == PSK_OK:
+            return False, None, None
+        self._n.pop(rid, None)
+        _raise(rc)
+        return True, sa, lat.value
+
+    def wait(self, rid: int, timeout_ms: float = -1.0):
+        """Block until the request finishes; returns (sa, latency_ms)."""
+        lat = ctypes.c_double(0)
+        sa = np.empty(self._n.get(rid, 0), dtype=np.uint32)
+        rc = _lib.psk_server_wait(self._h, rid, sa.ctypes.data, ctypes.byref(lat), timeout_ms)
+        if rc != PSK_EBUSY:
+            self._n.pop(rid, None)
+        _raise(rc)
+        return sa, lat.value
+
+    def peak_bytes(self) -> int:
+        return int(_lib.psk_server_peak_bytes(self._h))
+
+    def pending(self) -> int:
+        return int(_lib.psk_server_pending(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.psk_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "psk_level8.hpp"
+#include "psk_server.hpp"
 
 using namespace psk;
 
@@ -509,6 +510,186 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
             d.a->reset(mk);
         });
     });
+}
+
+// ---- serving ----------------------------------------------------------------
+
+uint64_t psk_server_footprint(uint64_t bytes, int batched) {
+    // text + SA + level workspace, plus the text-id partition of a batch
+    return psk_workspace_bytes8(bytes) + 5 * bytes + (batched ? 24 * bytes : 0);
+}
+
+static void server_worker(psk_server *s, int w) {
+    psk_ctx *c = s->ctxs[w];
+    for (;;) {
+        std::vector<SrvReq *> job;
+        u64 foot = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                if (!s->pending.empty()) {
+                    SrvReq *head = s->reqs[s->pending.front()].get();
+                    std::vector<SrvReq *> cand{head};
+                    u64 tot = head->n;
+                    if (head->n <= psk_server::kSmall) {
+                        for (size_t i = 1; i < s->pending.size(); i++) {
+                            SrvReq *r = s->reqs[s->pending[i]].get();
+                            if (r->n > psk_server::kSmall || tot + r->n > psk_server::kBatchBytes) break;
+                            cand.push_back(r);
+                            tot += r->n;
+                        }
+                    }
+                    u64 f = psk_server_footprint(tot, cand.size() > 1);
+                    if (s->budget == 0 || s->inflight == 0 || s->inflight + f <= s->budget) {
+                        for (size_t i = 0; i < cand.size(); i++) s->pending.pop_front();
+                        s->inflight += f;
+                        foot = f;
+                        job.swap(cand);
+                        break;
+                    }
+                }
+                s->cv_work.wait(lk);
+            }
+        }
+        int rc;
+        if (job.size() == 1) {
+            SrvReq *r = job[0];
+            rc = psk_psacak_inplace(c, r->text.data(), r->n, r->sa.data(), r->n);
+        } else {
+            std::vector<const uint8_t *> tp;
+            std::vector<uint64_t> lp;
+            std::vector<uint32_t *> sp;
+            for (SrvReq *r : job) {
+                tp.push_back(r->text.data());
+                lp.push_back(r->n);
+                sp.push_back(r->sa.data());
+            }
+            rc = psk_batch8(c, tp.data(), lp.data(), sp.data(), job.size());
+        }
+        double t = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (SrvReq *r : job) {
+                r->status = rc;
+                r->done = true;
+                r->t_done = t;
+                std::vector<u8>().swap(r->text);
+            }
+            s->inflight -= foot;
+            u64 res = 0;
+            for (psk_ctx *x : s->ctxs) res += x->arena.reserved();
+            s->peak = std::max(s->peak, res);
+            if (s->budget && res > s->budget) (void)psk_ctx_trim(c);
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+int psk_server_create(psk_server **out, uint64_t hbm_budget_bytes, int max_workers) {
+    if (!out || max_workers < 1 || max_workers > 8) return PSK_EINVAL;
+    psk_server *s = new (std::nothrow) psk_server();
+    if (!s) return PSK_ENOMEM;
+    s->budget = hbm_budget_bytes;
+    s->nworkers = max_workers;
+    for (int w = 0; w < max_workers; w++) {
+        psk_ctx *c = nullptr;
+        int rc = psk_ctx_create(&c, hbm_budget_bytes);
+        if (rc != PSK_OK) {
+            for (psk_ctx *x : s->ctxs) psk_ctx_destroy(x);
+            delete s;
+            return rc;
+        }
+        s->ctxs.push_back(c);
+    }
+    for (int w = 0; w < max_workers; w++) s->workers.emplace_back(server_worker, s, w);
+    *out = s;
+    return PSK_OK;
+}
+
+int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *id) {
+    if (!s || !id || (n && !text)) return PSK_EINVAL;
+    if (n >= 0xFFFFFFFFull) return PSK_ETOOLONG;
+    u64 f = psk_server_footprint(n, 0);
+    if (s->budget && f > s->budget) return PSK_EBUDGET;
+    std::unique_ptr<SrvReq> r(new (std::nothrow) SrvReq());
+    if (!r) return PSK_ENOMEM;
+    try {
+        r->text.assign(text, text + n);
+        r->sa.resize(n);
+    } catch (...) {
+        return PSK_ENOMEM;
+    }
+    r->n = n;
+    r->t_submit = now_ms();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        *id = r->id;
+        s->pending.push_back(r->id);
+        s->reqs[r->id] = std::move(r);
+    }
+    s->cv_work.notify_all();
+    return PSK_OK;
+}
+
+static int take_result(psk_server *s, std::unique_ptr<SrvReq> &r, uint32_t *sa, double *lat) {
+    if (sa && r->n) memcpy(sa, r->sa.data(), r->n * sizeof(u32));
+    if (lat) *lat = r->t_done - r->t_submit;
+    int st = r->status;
+    s->reqs.erase(r->id);
+    return st;
+}
+
+int psk_server_poll(psk_server *s, uint64_t id, int *done, uint32_t *sa, double *latency_ms) {
+    if (!s || !done) return PSK_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return PSK_ENOTFOUND;
+    *done = it->second->done ? 1 : 0;
+    if (!*done) return PSK_OK;
+    return take_result(s, it->second, sa, latency_ms);
+}
+
+int psk_server_wait(psk_server *s, uint64_t id, uint32_t *sa, double *latency_ms,
+                    double timeout_ms) {
+    if (!s) return PSK_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return PSK_ENOTFOUND;
+    SrvReq *r = it->second.get();
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+        return PSK_EBUSY;
+    return take_result(s, s->reqs.find(id)->second, sa, latency_ms);
+}
+
+uint64_t psk_server_peak_bytes(psk_server *s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->peak;
+}
+
+uint64_t psk_server_pending(psk_server *s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    u64 k = 0;
+    for (auto &kv : s->reqs) k += !kv.second->done;
+    return k;
+}
+
+void psk_server_destroy(psk_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (psk_ctx *c : s->ctxs) psk_ctx_destroy(c);
+    delete s;
 }
 
 }  // extern "C"

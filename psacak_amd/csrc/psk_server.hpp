@@ -1,0 +1,58 @@
+// psk_server.hpp -- continuous-batching request server (C ABI psk_server_*).
+//
+// Requests arrive at any time (psk_server_submit copies the text).  Worker
+// threads, each with its own context and HIP stream, take admitted work in
+// arrival order: consecutive small requests are coalesced into one batch
+// (one shared level-0 run, batch8_run), a large request runs alone.  A job is
+// admitted only while the estimated device footprint of all jobs in flight
+// stays inside the HBM budget, so requests that arrive while others are on
+// the GPU start as soon as a worker and memory are free, and jobs of
+// different workers overlap on the device.  Results are identical to a
+// single call's: batching never changes an SA (every text keeps its own
+// sentinel).
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+struct psk_server;
+
+namespace psk {
+
+inline double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct SrvReq {
+    u64 id = 0, n = 0, foot = 0;
+    std::vector<u8> text;
+    std::vector<u32> sa;
+    double t_submit = 0, t_done = 0;
+    int status = PSK_OK;
+    bool done = false;
+};
+
+}  // namespace psk
+
+struct psk_server {
+    psk::u64 budget = 0;
+    int nworkers = 1;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<psk::u64> pending;
+    std::unordered_map<psk::u64, std::unique_ptr<psk::SrvReq>> reqs;
+    psk::u64 next_id = 1, inflight = 0, peak = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::vector<psk_ctx *> ctxs;
+    // requests up to this size are coalesced; a coalesced batch holds at
+    // most kBatchBytes of text
+    static constexpr psk::u64 kSmall = 16ull << 20;
+    static constexpr psk::u64 kBatchBytes = 512ull << 20;
+};

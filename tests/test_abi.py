@@ -90,3 +90,27 @@ def test_no_device_is_an_error_not_an_abort():
     with pytest.raises(P.PskError) as e:
         P.psacak(b"mississippi")
     assert e.value.code == P.PSK_EDEVICE
+
+
+def test_server_admission_without_gpu():
+    with P.Server(hbm_budget=1 << 30, workers=2) as s:
+        # alone larger than the budget: rejected at submit, before any device work
+        big = P.Server.footprint(1 << 30)
+        assert big > 1 << 30
+        with pytest.raises(P.PskError) as e:
+            s._n  # noqa: B018
+            rc = P._lib.psk_server_submit(s._h, np.zeros(8, np.uint8).ctypes.data, 1 << 30,
+                                          ctypes.byref(ctypes.c_uint64()))
+            P._raise(rc)
+        assert e.value.code == P.PSK_EBUDGET
+        assert s.pending() == 0
+    assert P.Server.footprint(1000, True) > P.Server.footprint(1000, False)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_server_reports_device_error_without_gpu():
+    with P.Server(workers=1) as s:
+        rid = s.submit(b"mississippi")
+        with pytest.raises(P.PskError) as e:
+            s.wait(rid, timeout_ms=30000)
+        assert e.value.code == P.PSK_EDEVICE

@@ -220,6 +220,37 @@ def test_batch_matches_single_calls_on_repeats(ctx):
         assert np.array_equal(o, oracle.sacak8(x))
 
 
+def test_server_results_equal_alone_and_in_the_mix(ctx):
+    import threading
+    import time
+    note("server mix")
+    rng = np.random.default_rng(17)
+    L = gen.lengths(120, 8, 400000, seed=33)
+    texts = [gen.markov_dna(int(n), seed=200 + i) for i, n in enumerate(L)]
+    texts += [b"banana", b"", b"a", gen.iid_dna(3 << 20, seed=1)]
+    alone = [P.psacak(t, ctx) for t in texts]
+    with P.Server(hbm_budget=64 << 30, workers=3) as srv:
+        ids = {}
+
+        def producer():
+            for i, t in enumerate(texts):
+                ids[i] = srv.submit(t)
+                time.sleep(float(rng.exponential(0.0005)))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        lat = []
+        for i, t in enumerate(texts):
+            sa, ms = srv.wait(ids[i], timeout_ms=120000)
+            lat.append(ms)
+            assert np.array_equal(sa, alone[i]), i
+        assert srv.peak_bytes() > 0
+        assert srv.pending() == 0
+    for i in range(0, len(texts), 17):
+        assert np.array_equal(alone[i], oracle.sacak8(texts[i]))
+
+
 def test_budget_is_enforced():
     note("budget")
     with P.Context(hbm_budget=1 << 20) as c:
