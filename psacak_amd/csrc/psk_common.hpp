@@ -71,6 +71,16 @@ __device__ __forceinline__ T wave_inclusive_sum(T x) {
     return x;
 }
 
+// Maximum over the lanes of the wave (all lanes must be active).
+__device__ __forceinline__ u32 wave_max(u32 x) {
+    #pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        u32 y = __shfl_xor(x, o, WAVE);
+        x = x > y ? x : y;
+    }
+    return x;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_max(T x) {
     const int l = lane_id();

@@ -40,15 +40,30 @@ struct QPos {
     __device__ __forceinline__ u64 operator()(u64 j) const { return (u64)((i64)base + dir * (i64)j); }
 };
 
-constexpr u32 CHAIN_SMALL = 8192;
-
 // Predecessor of suffix x: none for x = 0 and, in a batch of concatenated
-// texts, none at a text's first position (start bit vector SB).
+// texts, none at a text's first position (start bit vector SB).  DB is the
+// "decided" mask (bit p: T[p] != T[p+1], or p ends a text).
 struct Pred {
     const u8 *T;
     const u32 *SB;
+    const u32 *DB;
     __device__ __forceinline__ bool has(u32 x) const { return x > 0 && !(SB && getbit(SB, x)); }
     __device__ __forceinline__ u32 byte(u32 x) const { return has(x) ? (u32)T[x - 1] : 0u; }
+    // number of positions before x holding T[x] in an unbroken run: the gap
+    // from x-1 down to the nearest decided position (a text start is preceded
+    // by a text end, which is decided)
+    __device__ __forceinline__ u32 run_before(u32 x) const {
+        if (x == 0) return 0;
+        u32 q = x - 1;
+        u32 w = q >> 5;
+        u32 m = DB[w] & (0xFFFFFFFFu >> (31 - (q & 31)));
+        while (!m) {
+            if (w == 0) return x;  // run reaches position 0
+            m = DB[--w];
+        }
+        u32 top = w * 32 + (31 - __clz(m));
+        return q - top;
+    }
 };
 
 // Every queue entry x gets its predecessor byte cached in dig[j] (j = read
@@ -56,90 +71,117 @@ struct Pred {
 // multi-split that follows needs T[x-1] as its digit.  A suffix without a
 // predecessor caches 0; consumers test Pred::has themselves.
 
-// One workgroup finishes a queue: generations until the queue is full (limit)
-// or a generation is empty.  A lone chain is followed by one thread.
-__global__ void k_chain_finish(Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c, u64 gs, u64 ge,
-                               u64 limit, u32 *err) {
-    __shared__ u32 lds[NWAVES + 1];
-    while (gs < ge) {
-        u64 cnt = ge - gs;
-        if (cnt == 1) {
-            // a lone chain: follow it serially
-            if (threadIdx.x == 0) {
-                u64 j = gs;
-                u32 x = SA[qp(j)];
-                for (;;) {
-                    u32 cp = pr.byte(x);
-                    dig[j] = (u8)cp;
-                    if (!(pr.has(x) && cp == c)) break;
-                    if (j + 1 >= limit) { atomicOr(err, 1u); break; }
-                    SA[qp(j + 1)] = x - 1;
-                    x--;
-                    j++;
-                }
-            }
-            __syncthreads();
-            return;
-        }
-        u64 newend = ge;
-        for (u64 j0 = gs; j0 < ge; j0 += BLOCK) {
-            u64 j = j0 + threadIdx.x;
-            u32 x = 0;
-            bool v = false;
-            if (j < ge) {
-                x = SA[qp(j)];
-                u32 cp = pr.byte(x);
-                dig[j] = (u8)cp;
-                v = pr.has(x) && cp == c;
-            }
-            u32 tot;
-            u32 ex = block_exclusive_sum((u32)v, lds, &tot);
-            if (v) {
-                if (newend + ex < limit) SA[qp(newend + ex)] = x - 1;
-                else atomicOr(err, 1u);
-            }
-            newend += tot;
-        }
-        gs = ge;
-        ge = newend;
-        __syncthreads();
-    }
-}
-
-// Fill one bucket's queue: positions qp(0..limit); generation 0 is
-// [0, *d_start).  Caches every entry's predecessor byte in dig[0..limit).
-// d_tmp: one device u32.
+// Fill one bucket's queue: positions qp(0..limit); generation 0 (the entries
+// induced from other buckets) is [0, *d_start).  Every further entry is a
+// chain: seed x_k followed by x_k-1, x_k-2, ... while the predecessor byte is
+// c, so seed k contributes r_k entries, its run of c's before it.  The queue
+// is breadth-first: all depth-1 entries in seed order, then depth 2, ...;
+// the entry (k, depth t) sits at a + sum_{t'<t} #{r >= t'} + #{k' < k :
+// r_k' >= t}, i.e. the entries emitted seed by seed, stably sorted by depth.
+// One pass measures the runs, one scan gives emission offsets, and one stable
+// multi-split by depth places them (two passes if a run exceeds 256).
+// Caches every entry's predecessor byte in dig[0..limit).  d_tmp: 2 u32.
 inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
                          const u32 *d_start, u64 limit, u32 *d_tmp) {
-    u64 gs = 0, ge = d.read_u32(d_start);
+    const u64 a = d.read_u32(d_start);
     u32 *err = d.err;
-    while (ge > gs) {
-        u64 cnt = ge - gs;
-        if (cnt <= CHAIN_SMALL) {
-            k_chain_finish<<<1, BLOCK, 0, d.s>>>(pr, SA, dig, qp, c, gs, ge, limit, err);
-            PSK_LAUNCH_CHECK();
-            return;
-        }
-        const u64 g0 = gs, e0 = ge, lim = limit;
-        // gather the predecessor bytes of this generation (the only gather)
-        int prec = d.pbegin("chain8_gather", 9.0 * cnt);
-        for_each(d, cnt, [=] __device__(u64 i) { dig[g0 + i] = (u8)pr.byte(SA[qp(g0 + i)]); });
+    if (a > limit) fail(PSK_EDEVICE, "queue seed count exceeds its bucket part");
+    Arena::Mark mk = d.a->mark();
+    u32 *run = d.a->get<u32>(a + 1);
+    u8 *stop = d.a->get<u8>(a + 1);
+    u32 *maxr = d_tmp;
+    d.zero(maxr, sizeof(u32));
+    const u64 nchain = limit - a;
+    {
+        int prec = d.pbegin("chain8_runs", 10.0 * a);
+        const u64 aa = a;
+        // one thread per seed; the grid is a multiple of the wave so every
+        // lane takes part in the wave-wide maximum (one atomic per wave)
+        u64 blocks = (aa + BLOCK - 1) / BLOCK;
+        k_for_each_waves<<<(u32)std::max<u64>(blocks, 1), BLOCK, 0, d.s>>>(
+            [=] __device__(u64 k, bool live) {
+                u32 r = 0;
+                if (live) {
+                    u32 x = SA[qp(k)];
+                    u32 cp = pr.byte(x);
+                    dig[k] = (u8)cp;
+                    u32 sb = cp;
+                    if (pr.has(x) && cp == c) {
+                        r = pr.run_before(x);  // T[x] == c: the run of c's before x
+                        sb = pr.byte(x - r);   // predecessor byte of the chain's last entry
+                    }
+                    run[k] = r;
+                    stop[k] = (u8)sb;
+                }
+                // same-address atomics serialise in L2: only raise the maximum
+                // when this wave beats the value already stored
+                u32 wm = wave_max(r);
+                if (lane_id() == 0 && wm > __atomic_load_n(maxr, __ATOMIC_RELAXED))
+                    atomicMax(maxr, wm);
+            },
+            aa);
+        PSK_LAUNCH_CHECK();
         d.pend(prec);
-        if (ge >= limit) return;  // queue full: this was the last generation
-        scan_exclusive(
-            d, cnt,
-            [=] __device__(u64 i) { return (u32)(dig[g0 + i] == c && pr.has(SA[qp(g0 + i)])); },
-            [=] __device__(u64 i, u32 off, u32 v) {
-                if (v) {
-                    if (e0 + off < lim) SA[qp(e0 + off)] = SA[qp(g0 + i)] - 1;
-                    else atomicOr(err, 2u);
+    }
+    if (nchain == 0) { d.a->reset(mk); return; }
+    // emission offsets
+    u32 *eo = d.a->get<u32>(a + 1);
+    u32 *tot = d_tmp + 1;
+    scan_exclusive(
+        d, a, [=] __device__(u64 k) { return run[k]; },
+        [=] __device__(u64 k, u32 ex, u32) { eo[k] = ex; }, tot);
+    u32 hv[2];
+    d.read(hv, d_tmp, sizeof(hv));
+    const u32 R = hv[0];
+    if ((u64)hv[1] != nchain) fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
+    // emit (depth-1, entry, its predecessor byte) seed by seed
+    u32 *ek = d.a->get<u32>(nchain), *ev = d.a->get<u32>(nchain);
+    u8 *eb = d.a->get<u8>(nchain);
+    for_each(d, a, [=] __device__(u64 k) {
+        u32 r = run[k];
+        if (!r) return;
+        u32 x = SA[qp(k)], o = eo[k];
+        for (u32 t = 1; t <= r; t++) {
+            ek[o + t - 1] = t - 1;
+            ev[o + t - 1] = x - t;
+            eb[o + t - 1] = t < r ? (u8)c : stop[k];
+        }
+    });
+    const u64 lim = limit, base = a;
+    if (R <= 256) {
+        multisplit(
+            d, nchain,
+            [=] __device__(u64 j, u32 *key) {
+                *key = ek[j];
+                return true;
+            },
+            [=] __device__(u64 j, u32, u32 gpos, u32) {
+                u64 q = base + gpos;
+                if (q < lim) {
+                    SA[qp(q)] = ev[j];
+                    dig[q] = eb[j];
+                } else {
+                    atomicOr(err, 2u);
                 }
             },
-            d_tmp);
-        u32 produced = d.read_u32(d_tmp);
-        gs = ge;
-        ge += produced;
+            nullptr);
+    } else {
+        // long runs: stable LSD sort of (depth, emission index)
+        u64 *k0 = d.a->get<u64>(nchain), *k1 = d.a->get<u64>(nchain);
+        u32 *v0 = d.a->get<u32>(nchain), *v1 = d.a->get<u32>(nchain);
+        for_each(d, nchain, [=] __device__(u64 j) {
+            k0[j] = ek[j];
+            v0[j] = (u32)j;
+        });
+        bool sw = radix_sort_kv(d, nchain, k0, v0, k1, v1, bit_width(R - 1));
+        const u32 *V = sw ? v1 : v0;
+        for_each(d, nchain, [=] __device__(u64 i) {
+            u32 j = V[i];
+            SA[qp(base + i)] = ev[j];
+            dig[base + i] = eb[j];
+        });
     }
+    d.a->reset(mk);
 }
 
 // Per-bucket device tables: head, tail, fill (induce counters).
@@ -152,8 +194,9 @@ struct Induce8Tables {
 // Batches (SB != nullptr): K texts concatenated, ends[0..K) their last
 // positions; the L-scan starts from every text's sentinel, in text order.
 inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Induce8Tables tb,
-                    u8 *dig, const u32 *SB = nullptr, const u32 *ends = nullptr, u64 K = 1) {
-    const Pred pr{T, SB};
+                    u8 *dig, const u32 *DB, const u32 *SB = nullptr, const u32 *ends = nullptr,
+                    u64 K = 1) {
+    const Pred pr{T, SB, DB};
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
     u32 *err = d.err;
     const u64 nn = n;

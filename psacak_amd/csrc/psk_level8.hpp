@@ -35,7 +35,8 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     const u64 nwords = (n + 31) / 32;
     u32 *sbits = d.a->get<u32>(nwords + 1);
     u32 *lmsbits = d.a->get<u32>(nwords + 1);
-    compute_types(d, T, n, sbits, EB);
+    u32 *dbits = d.a->get<u32>(nwords + 1);
+    compute_types(d, T, n, sbits, EB, dbits);
 
     // bucket statistics (Buckets::new psacak8.rs:355-374)
     u32 *stat = d.a->get<u32>(3 * 256 + 1);
@@ -101,7 +102,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             SA[lmsbase[key] + (gpos - dstart)] = lms_text[i];
         },
         nullptr);
-    induce8(d, T, n, SA, B, tb, dig, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, dbits, SB, ENDS, K);
 
     // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
     u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
@@ -121,7 +122,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         u32 p = SA1[j];
         SA[j + lmsoff[T[p]]] = p;
     });
-    induce8(d, T, n, SA, B, tb, dig, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, dbits, SB, ENDS, K);
     d.a->reset(mk);
 }
 

@@ -60,6 +60,14 @@ __global__ void k_for_each(F f, u64 n) {
     for (; i < n; i += stride) f(i);
 }
 
+// One item per thread, every thread of the grid calls f(i, i < n): for
+// kernels that use wave-wide operations.
+template <typename F>
+__global__ void k_for_each_waves(F f, u64 n) {
+    u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    f(i, i < n);
+}
+
 // Launch f(i) for i in [0, n) with a grid-stride loop.
 template <typename F>
 inline void for_each(Dev &d, u64 n, F f) {

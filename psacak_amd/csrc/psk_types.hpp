@@ -202,7 +202,7 @@ __global__ void k_type_carry(const u8 *blk_has, const u8 *blk_first, u32 nb, u8 
 
 template <class C>
 __global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits, u64 nwords,
-                             const u32 *endbits) {
+                             const u32 *endbits, u32 *dbits) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
     u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
     u32 D[TY_WPT], V[TY_WPT];
@@ -221,13 +221,19 @@ __global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits,
     for (int k = TY_WPT - 1; k >= 0; k--) {
         u32 S = resolve_word(D[k], V[k], c);
         c = S & 1u;
-        if (w0 + k < nwords) sbits[w0 + k] = S;
+        if (w0 + k < nwords) {
+            sbits[w0 + k] = S;
+            if (dbits) dbits[w0 + k] = D[k];
+        }
     }
 }
 
 // S-type bit vector of t[0..n) into sbits[0 .. ceil(n/32)).
+// dbits (optional): bit p set where T[p] != T[p+1] or p ends a text (the
+// "decided" mask); run lengths of equal symbols are read from it.
 template <class C>
-inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endbits = nullptr) {
+inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endbits = nullptr,
+                          u32 *dbits = nullptr) {
     if (n == 0) return;
     u64 nwords = (n + 31) / 32;
     u32 nb = ceil_div(nwords, (u64)BLOCK * TY_WPT);
@@ -239,7 +245,7 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endb
     PSK_LAUNCH_CHECK();
     k_type_carry<<<1, BLOCK, 0, d.s>>>(bh, bf, nb, bc);
     PSK_LAUNCH_CHECK();
-    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords, endbits);
+    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords, endbits, dbits);
     PSK_LAUNCH_CHECK();
     d.a->reset(mk);
 }
