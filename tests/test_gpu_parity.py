@@ -251,6 +251,19 @@ def test_server_results_equal_alone_and_in_the_mix(ctx):
         assert np.array_equal(alone[i], oracle.sacak8(texts[i]))
 
 
+def test_cli_matches_crate_output_format(tmp_path, capsys):
+    from psacak_amd import cli
+    note("cli")
+    f = tmp_path / "in.txt"
+    f.write_bytes(b"mississippi" * 1000)
+    assert cli.main(["-t", "2", "-c", str(f)]) == 0
+    err = capsys.readouterr().err.splitlines()
+    assert err[0] == f"load 11000 bytes from `{f}`"
+    assert err[1].startswith(" time: ") and err[1].count("s ") == 2
+    assert err[2].startswith("  rss: ") and err[2].endswith("MiB")
+    assert err[3] == "check: true"
+
+
 def test_budget_is_enforced():
     note("budget")
     with P.Context(hbm_budget=1 << 20) as c:
