@@ -62,6 +62,173 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
         nullptr);
 }
 
+// ---- fused relabel + compaction of one sorted round --------------------------
+// For sorted keys K (hi = old label, lo = successor label) and items V:
+//   g0(i) = first index of i's old group, g1(i) = its last index,
+//   s0(i) = first index of i's subgroup (items with an equal key),
+//   RK[V[i]] = hi + (s0 - g0)      (first-slot labels), or
+//              hi - (g1 - s0)      (last-slot labels),
+//   keep(i)  = i's subgroup has >= 2 items -> compacted, in order, into nxt.
+// One reduce pass (per tile: max of group starts, max of subgroup starts,
+// number kept, min of group ends) and one apply pass replace five passes
+// and an index table.
+struct RelIn {
+    const u64 *K;
+    u64 A;
+    int B;
+    // flags of item i: group start, subgroup start, kept, group end
+    __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
+        u64 k = K[i], hi = k >> B;
+        u64 kp_ = i ? K[i - 1] : ~0ull;
+        u64 kn_ = i + 1 < A ? K[i + 1] : ~0ull;
+        gs = i == 0 || (kp_ >> B) != hi;
+        ss = i == 0 || kp_ != k;
+        ge = i + 1 == A || (kn_ >> B) != hi;
+        kp = !ss || (i + 1 < A && kn_ == k);
+    }
+};
+
+constexpr int REL_ROUNDS = 16;
+constexpr u64 REL_TILE = (u64)BLOCK * REL_ROUNDS;
+
+__global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te) {
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1];
+    u64 base = (u64)blockIdx.x * REL_TILE;
+    u32 g = 0, s = 0, k = 0, e = 0xFFFFFFFFu;
+    for (int r = 0; r < REL_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        if (i < in.A) {
+            bool gs, ss, kp, ge;
+            in.flags(i, gs, ss, kp, ge);
+            if (gs) g = (u32)i;
+            if (ss) s = (u32)i;
+            k += kp;
+            if (ge && (u32)i < e) e = (u32)i;
+        }
+    }
+    u32 a1, a2, a3;
+    block_incl<OpMax>(g, l1, &a1);
+    block_incl<OpMax>(s, l2, &a2);
+    block_incl<OpSum>(k, l3, &a3);
+    // min of group ends: minimum over the block
+    u32 x = e;
+    #pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        u32 y = __shfl_xor(x, o, WAVE);
+        x = x < y ? x : y;
+    }
+    if (lane_id() == 0) l1[wave_id()] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 m = 0xFFFFFFFFu;
+        for (int w = 0; w < NWAVES; w++) m = m < l1[w] ? m : l1[w];
+        tg[blockIdx.x] = a1;
+        ts[blockIdx.x] = a2;
+        tk[blockIdx.x] = a3;
+        te[blockIdx.x] = m;
+    }
+}
+
+// single workgroup: a[i] = min(a[i+1 ..]) (exclusive suffix minimum)
+__global__ void k_rmin_tiles(u32 *a, u32 nt) {
+    __shared__ u32 wt[NWAVES];
+    __shared__ u32 carry_sh;
+    const int l = lane_id(), w = wave_id();
+    if (threadIdx.x == 0) carry_sh = 0xFFFFFFFFu;
+    __syncthreads();
+    for (i64 base = (i64)((nt - 1) / BLOCK) * BLOCK; base >= 0; base -= BLOCK) {
+        u64 i = (u64)base + threadIdx.x;
+        u32 v = i < nt ? a[i] : 0xFFFFFFFFu;
+        u32 x = v;
+        #pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            u32 y = __shfl_down(x, o, WAVE);
+            if (l + o < WAVE) x = x < y ? x : y;
+        }
+        if (l == 0) wt[w] = x;
+        __syncthreads();
+        u32 carry = carry_sh;
+        u32 after = carry;
+        for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
+        u32 xn = __shfl_down(x, 1, WAVE);
+        u32 ex = (l + 1 < WAVE) ? (xn < after ? xn : after) : after;
+        if (i < nt) a[i] = ex;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 m = carry;
+            for (int w2 = 0; w2 < NWAVES; w2++) m = m < wt[w2] ? m : wt[w2];
+            carry_sh = m;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32 *tg,
+                            const u32 *ts, const u32 *tk, const u32 *te, bool lastconv, u64 mm,
+                            u32 *err) {
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1];
+    __shared__ u32 wt[NWAVES];
+    __shared__ u32 g1sh[REL_TILE];
+    const u64 base = (u64)blockIdx.x * REL_TILE;
+    const int l = lane_id(), w = wave_id();
+    if (lastconv) {
+        // g1 of every tile item: inclusive suffix minimum of group ends,
+        // rounds right to left, seeded with the tiles to the right
+        u32 carry = te[blockIdx.x];
+        for (int r = REL_ROUNDS - 1; r >= 0; r--) {
+            u64 i = base + (u64)r * BLOCK + threadIdx.x;
+            u32 e = 0xFFFFFFFFu;
+            if (i < in.A) {
+                bool gs, ss, kp, ge;
+                in.flags(i, gs, ss, kp, ge);
+                if (ge) e = (u32)i;
+            }
+            u32 x = e;
+            #pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                u32 y = __shfl_down(x, o, WAVE);
+                if (l + o < WAVE) x = x < y ? x : y;
+            }
+            if (l == 0) wt[w] = x;
+            __syncthreads();
+            u32 after = carry;
+            for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
+            g1sh[r * BLOCK + threadIdx.x] = x < after ? x : after;
+            u32 rm = carry;
+            for (int w2 = 0; w2 < NWAVES; w2++) rm = rm < wt[w2] ? rm : wt[w2];
+            __syncthreads();
+            carry = rm;
+        }
+    }
+    u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x];
+    for (int r = 0; r < REL_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        bool gs = false, ss = false, kp = false, ge = false;
+        if (i < in.A) in.flags(i, gs, ss, kp, ge);
+        u32 a1, a2, a3;
+        u32 g0 = block_incl<OpMax>(gs ? (u32)i : 0u, l1, &a1);
+        u32 s0 = block_incl<OpMax>(ss ? (u32)i : 0u, l2, &a2);
+        u32 kk = block_incl<OpSum>(kp ? 1u : 0u, l3, &a3);
+        g0 = g0 > rg ? g0 : rg;
+        s0 = s0 > rs ? s0 : rs;
+        kk += rk;
+        if (i < in.A) {
+            u64 hi = in.K[i] >> in.B;
+            if (hi > mm) {
+                atomicOr(err, 32u);
+            } else {
+                u32 p = V[i];
+                RK[p] = lastconv ? (u32)hi - (g1sh[r * BLOCK + threadIdx.x] - s0)
+                                 : (u32)hi + (s0 - g0);
+                if (kp) nxt[kk - 1] = p;
+            }
+        }
+        rg = rg > a1 ? rg : a1;
+        rs = rs > a2 ? rs : a2;
+        rk += a3;
+    }
+}
+
 // Sort items[0..A) by chain keys (see above); RK holds the labels of every
 // position 0..m (m = sentinel) and receives the final labels of the items.
 // If last_slot_labels, the items' initial group labels name the group's last
@@ -76,7 +243,6 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     u32 *vt = d.a->get<u32>(A);
     u64 *keys = d.a->get<u64>(A);
     u64 *kt = d.a->get<u64>(A);
-    u32 *fidx = d.a->get<u32>(m + 1);
     u32 *cnt = d.a->get<u32>(1);
     PSK_HIP(hipMemcpyAsync(act, items, (u64)A * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
     const int B = bit_width(m);
@@ -100,34 +266,26 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
         const u64 AA = A;
-        // groups are contiguous in sorted order: record each group's first
-        // (or, for last-slot labels, last) sorted index with a plain store
-        const bool lastconv = last_slot_labels && rounds == 1;
-        for_each(d, AA, [=] __device__(u64 i) {
-            u64 hi = K[i] >> B;
-            if (hi > mm) { atomicOr(err, 32u); return; }
-            bool edge = lastconv ? (i + 1 == AA || (K[i + 1] >> B) != hi)
-                                 : (i == 0 || (K[i - 1] >> B) != hi);
-            if (edge) fidx[hi] = (u32)i;
-        });
-        // new label = first slot of the item's subgroup + 1
-        scan_inclusive_max(
-            d, AA, [=] __device__(u64 i) { return (i == 0 || K[i] != K[i - 1]) ? (u32)i : 0u; },
-            [=] __device__(u64 i, u32 s0, u32) {
-                u64 hi = K[i] >> B;
-                if (hi > mm) return;
-                RK[V[i]] = lastconv ? (u32)hi - (fidx[hi] - s0) : (u32)hi + (s0 - fidx[hi]);
-            });
-        scan_exclusive(
-            d, AA,
-            [=] __device__(u64 i) {
-                bool keep = (i > 0 && K[i] == K[i - 1]) || (i + 1 < AA && K[i] == K[i + 1]);
-                return (u32)keep;
-            },
-            [=] __device__(u64 i, u32 off, u32 v) {
-                if (v) nxt[off] = V[i];
-            },
-            cnt);
+        {
+            const bool lastconv = last_slot_labels && rounds == 1;
+            Arena::Mark m3 = d.a->mark();
+            const u32 nt = ceil_div(AA, REL_TILE);
+            u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
+            u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
+            RelIn in{K, AA, B};
+            k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te);
+            PSK_LAUNCH_CHECK();
+            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
+            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
+            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, cnt);
+            if (lastconv) k_rmin_tiles<<<1, BLOCK, 0, d.s>>>(te, nt);
+            PSK_LAUNCH_CHECK();
+            int pr = d.pbegin("chain_relabel", 28.0 * AA);
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, RK, nxt, tg, ts, tk, te, lastconv, mm, err);
+            PSK_LAUNCH_CHECK();
+            d.pend(pr);
+            d.a->reset(m3);
+        }
         A = d.read_u32(cnt);
         if (nxt != act) std::swap(act, vt);
         h *= 2;

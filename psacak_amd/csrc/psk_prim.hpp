@@ -355,18 +355,25 @@ inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char
 constexpr int RS_ROUNDS = 8;
 constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
 
-__global__ void k_rs_count(const u64 *__restrict__ ki, u64 n, int shift, u32 *hist, u32 nt) {
+__global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, u64 n, int shift,
+                                                    u32 *hist, u32 nt) {
     __shared__ u32 h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
     u64 base = (u64)blockIdx.x * RS_TILE;
+    u32 dk[RS_ROUNDS];
+    bool vv[RS_ROUNDS];
+    // all loads first (independent, in flight together)
     #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        bool v = i < n;
-        u32 key = v ? (u32)(ki[i] >> shift) & 255u : 0u;
-        u64 peers = match_key8(v, key);
-        if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[key], (u32)__popcll(peers));
+        vv[r] = i < n;
+        dk[r] = vv[r] ? (u32)(ki[i] >> shift) & 255u : 0u;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        u64 peers = match_key8(vv[r], dk[r]);
+        if (vv[r] && (peers & lanemask_lt()) == 0) atomicAdd(&h[dk[r]], (u32)__popcll(peers));
     }
     __syncthreads();
     hist[(u64)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
@@ -374,65 +381,84 @@ __global__ void k_rs_count(const u64 *__restrict__ ki, u64 n, int shift, u32 *hi
 
 // off: exclusive scan of the digit-major count matrix (256*nt + 1 entries,
 // the last one the total), so the count of (digit d, tile t) is
-// off[d*nt+t+1] - off[d*nt+t].
+// off[d*nt+t+1] - off[d*nt+t].  Ranking: every round's wave-level digit
+// counts go to their own LDS slice, one pass per digit turns them into
+// tile-local offsets in (round, wave) order, and the tile is staged in LDS
+// in digit order before the coalesced write-out.
 __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki,
                                                       const u32 *__restrict__ vi,
                                                       u64 *__restrict__ ko, u32 *__restrict__ vo,
                                                       u64 n, int shift, const u32 *off, u32 nt) {
     __shared__ u64 sk[RS_TILE];
     __shared__ u32 sv[RS_TILE];
-    __shared__ u32 tstart[256], gbase[256], run[256];
-    __shared__ u32 wc[NWAVES][256];
+    __shared__ u32 tstart[256], gbase[256];
+    __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
     __shared__ u32 red[NWAVES + 1];
     const u32 t = blockIdx.x;
     const u32 dg = threadIdx.x;
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        #pragma unroll
+        for (int w = 0; w < NWAVES; w++) wcnt[r][w][dg] = 0;
     {
         u64 o = (u64)dg * nt + t;
         u32 g = off[o];
         u32 c = off[o + 1] - g;
         u32 tot;
-        u32 ex = block_exclusive_sum(c, red, &tot);
+        u32 ex = block_exclusive_sum(c, red, &tot);  // includes a barrier
         tstart[dg] = ex;
         gbase[dg] = g;
-        run[dg] = 0;
-        #pragma unroll
-        for (int w = 0; w < NWAVES; w++) wc[w][dg] = 0;
     }
-    __syncthreads();
     const u64 base = (u64)t * RS_TILE;
     const int w = wave_id();
-    #pragma unroll 1
+    u64 key[RS_ROUNDS];
+    u32 val[RS_ROUNDS];
+    u32 dk[RS_ROUNDS];
+    unsigned short rk[RS_ROUNDS];
+    bool vv[RS_ROUNDS];
+    #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        bool v = i < n;
-        u64 key = v ? ki[i] : 0ull;
-        u32 val = v ? vi[i] : 0u;
-        u32 dk = (u32)(key >> shift) & 255u;
-        u64 peers = match_key8(v, dk);
-        u32 rk = (u32)__popcll(peers & lanemask_lt());
-        if (v && rk == 0) wc[w][dk] = (u32)__popcll(peers);
-        __syncthreads();
-        if (v) {
-            u32 lp = tstart[dk] + run[dk] + rk;
-            for (int w2 = 0; w2 < w; w2++) lp += wc[w2][dk];
-            sk[lp] = key;
-            sv[lp] = val;
-        }
-        __syncthreads();
-        {
-            u32 s = 0;
-            #pragma unroll
-            for (int w2 = 0; w2 < NWAVES; w2++) { s += wc[w2][dg]; wc[w2][dg] = 0; }
-            run[dg] += s;
-        }
-        __syncthreads();
+        vv[r] = i < n;
+        key[r] = vv[r] ? ki[i] : 0ull;
+        val[r] = vv[r] ? vi[i] : 0u;
     }
+    __syncthreads();  // wcnt zeroed
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        dk[r] = (u32)(key[r] >> shift) & 255u;
+        u64 peers = match_key8(vv[r], dk[r]);
+        rk[r] = (unsigned short)__popcll(peers & lanemask_lt());
+        if (vv[r] && rk[r] == 0) wcnt[r][w][dk[r]] = (unsigned short)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        u32 run = tstart[dg];
+        #pragma unroll
+        for (int r = 0; r < RS_ROUNDS; r++)
+            #pragma unroll
+            for (int w2 = 0; w2 < NWAVES; w2++) {
+                u32 c = wcnt[r][w2][dg];
+                wcnt[r][w2][dg] = (unsigned short)run;
+                run += c;
+            }
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        if (vv[r]) {
+            u32 lp = (u32)wcnt[r][w][dk[r]] + rk[r];
+            sk[lp] = key[r];
+            sv[lp] = val[r];
+        }
+    }
+    __syncthreads();
     u64 cnt = n - base < RS_TILE ? n - base : RS_TILE;
     for (u32 j = threadIdx.x; j < cnt; j += BLOCK) {
-        u64 key = sk[j];
-        u32 dk = (u32)(key >> shift) & 255u;
-        u64 g = (u64)gbase[dk] + (j - tstart[dk]);
-        ko[g] = key;
+        u64 k = sk[j];
+        u32 d = (u32)(k >> shift) & 255u;
+        u64 g = (u64)gbase[d] + (j - tstart[d]);
+        ko[g] = k;
         vo[g] = sv[j];
     }
 }
