@@ -313,44 +313,69 @@ inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
 }
 
 // Labels of LMS suffixes placed at their bucket tails in the order of list
-// (grouped by symbol value, e.g. sorted by it): the last of a group gets slot
-// s[x] = tail-1, the one before it tail-2, ...  grp: scratch of m+1 u32.
-inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, u64 cnt, u32 *RK,
-                                u32 *grp) {
+// (grouped by symbol value, ascending, e.g. sorted by it): the last of a group
+// gets slot s[x] = tail-1, the one before it tail-2, ...  vals: the symbol of
+// each list item when the caller has it (sorted keys), else nullptr (one
+// gather per item).  grp: scratch of m+1 u32.
+inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64 *vals, u64 cnt,
+                                u32 *RK, u32 *grp) {
+    if (cnt == 0) return;
     u32 *err = d.err;
-    for_each(d, cnt, [=] __device__(u64 j) {
-        u32 v = s[list[j]];
-        if (j + 1 == cnt || s[list[j + 1]] != v) grp[v] = (u32)j;
+    Arena::Mark mk = d.a->mark();
+    u32 *vb = vals ? nullptr : d.a->get<u32>(cnt);
+    // group ends: a wave compares neighbours by shuffle
+    for_each_waves(d, cnt, [=] __device__(u64 j, bool live) {
+        u32 v = live ? (vals ? (u32)vals[j] : s[list[j]]) : 0u;
+        u32 vn = __shfl_down(v, 1, WAVE);
+        if (lane_id() == WAVE - 1 && j + 1 < cnt) vn = vals ? (u32)vals[j + 1] : s[list[j + 1]];
+        if (live) {
+            if (vb) vb[j] = v;
+            if (j + 1 == cnt || vn != v) grp[v] = (u32)j;
+        }
     });
     for_each(d, cnt, [=] __device__(u64 j) {
-        u32 p = list[j], v = s[p];
+        u32 v = vals ? (u32)vals[j] : vb[j];
         u32 back = grp[v] - (u32)j;
-        if (back <= v) RK[p] = v - back + 1;
+        if (back <= v) RK[list[j]] = v - back + 1;
         else atomicOr(err, 256u);
     });
+    d.a->reset(mk);
 }
 
-void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth);
+void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
+             const u32 *map = nullptr);
 
-// Sorted LMS positions of a level (ties between equal LMS substrings in the
-// induced order), given complete labels RK of every position: write the SA
-// and keep the LMS entries.
-inline void write_sa_from_labels(Dev &d, u64 m, const u32 *RK, u32 *SA) {
+// SA of a level from complete labels RK (label = slot + 1); with map, the
+// entries are map[x] instead of x (the caller's positions of this level's
+// symbols: one sequential read instead of a later gather over the SA).
+inline void write_sa_from_labels(Dev &d, u64 m, const u32 *RK, u32 *SA, const u32 *map = nullptr) {
     u32 *err = d.err;
     for_each(d, m, [=] __device__(u64 x) {
         u32 r = RK[x];
-        if (r >= 1 && r <= m) SA[r - 1] = (u32)x;
+        if (r >= 1 && r <= m) SA[r - 1] = map ? map[x] : (u32)x;
         else atomicOr(err, 64u);
     });
 }
 
-inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lmsbits, u32 *SA,
-                               u32 *lms_sorted) {
-    write_sa_from_labels(d, m, RK, SA);
+// LMS positions in the order of their stage-1 labels (distinct slots
+// 1..m): each LMS position x marks its slot with x + 1 in a zeroed array of
+// m words, then one compaction keeps the marks in slot order.  lms_text is in
+// text order, so the label reads are monotone; only the m1 mark writes are
+// scattered.
+inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lms_text, u32 m1,
+                               u32 *mark, u32 *lms_sorted) {
+    u32 *err = d.err;
+    d.zero(mark, m * sizeof(u32));
+    for_each(d, m1, [=] __device__(u64 r) {
+        u32 x = lms_text[r];
+        u32 l = RK[x];
+        if (l >= 1 && l <= m) mark[l - 1] = x + 1;
+        else atomicOr(err, 64u);
+    });
     scan_exclusive(
-        d, m, [=] __device__(u64 i) { return getbit(lmsbits, SA[i]); },
+        d, m, [=] __device__(u64 i) { return mark[i] ? 1u : 0u; },
         [=] __device__(u64 i, u32 off, u32 v) {
-            if (v) lms_sorted[off] = SA[i];
+            if (v) lms_sorted[off] = mark[i] - 1;
         },
         nullptr);
 }
@@ -358,37 +383,42 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lmsbits,
 // Solve the reduced problem of a level: names -> SA1 (as LMS positions).
 template <class C>
 inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
-                          const u32 *R, u32 m1, u32 *SA1, int depth, LevelStat &st,
-                          const u32 *EB = nullptr) {
+                          const u32 *lmsbits, u64 nwords, u32 *NP, u32 m1, u32 *SA1, int depth,
+                          LevelStat &st, const u32 *EB = nullptr) {
     Dev &d = rt.d;
     if (m1 == 0) return;
     Arena::Mark mk = d.a->mark();
     u32 *reduced = d.a->get<u32>(m1);
     u32 *tails = d.a->get<u32>(m1);
-    u32 *diff = d.a->get<u32>(m1);
+    u8 *diff = d.a->get<u8>(m1);
     u32 *dk = d.a->get<u32>(1);
-    u32 k = name_lms(d, T, lms_sorted, lms_text, R, m1, diff, reduced, tails, dk, EB);
+    u32 k = name_lms(d, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, diff, reduced, tails, dk,
+                     EB);
     st.k = k;
     rt.levels[depth] = st;
+    // SA1 of the reduced string, written as LMS positions (lms_text maps a
+    // reduced index to its position)
     if (k < m1) {
         u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
         compute_types(d, (const u32 *)reduced, m1, sb);
         translate_names(d, reduced, m1, sb, tails);
-        level32(rt, reduced, m1, sb, SA1, depth + 1);
+        level32(rt, reduced, m1, sb, SA1, depth + 1, lms_text);
     } else {
-        inverse_names(d, reduced, m1, SA1);
+        inverse_names(d, reduced, m1, SA1, lms_text);
     }
-    permute_lms(d, SA1, m1, lms_text);
     d.a->reset(mk);
 }
 
 // One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
-inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth) {
+// map (optional): SA entries are written as map[x] (see write_sa_from_labels).
+inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
+                    const u32 *map) {
     Dev &d = rt.d;
     if (m == 0) return;
     if (m <= 3) {
         k_naive_small<u32><<<1, 1, 0, d.s>>>(s, (u32)m, SA);
         PSK_LAUNCH_CHECK();
+        if (map) permute_lms(d, SA, (u32)m, map);
         return;
     }
     Arena::Mark mk = d.a->mark();
@@ -400,8 +430,8 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     PSK_LAUNCH_CHECK();
     const u32 m1 = d.read_u32(ctr);
     u32 *lms_text = d.a->get<u32>(m1 + 1);
-    u32 *R = d.a->get<u32>(m / 2 + 1);
-    bits_to_positions(d, lmsbits, nwords, lms_text, R);
+    u32 *NP = d.a->get<u32>(m / 2 + 1);
+    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
 
     // L and S item lists (ascending positions)
     u32 *Litems = d.a->get<u32>(m);
@@ -437,14 +467,14 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
             v0[r] = lms_text[r];
         });
         bool sw = radix_sort_kv(d, m1, k0, v0, k1, v1, bit_width(m));
-        labels_lms_at_tails(d, s, sw ? v1 : v0, m1, RK, cnt);
+        labels_lms_at_tails(d, s, sw ? v1 : v0, sw ? k1 : k0, m1, RK, cnt);
         d.a->reset(m2);
     }
     chain_sort(rt, m, RK, Litems, nL, false);
     labels_S(d, s, m, sbits, RK);
     chain_sort(rt, m, RK, Sitems, nS, true);
     u32 *lms_sorted = d.a->get<u32>(m1 + 1);
-    extract_lms_sorted(d, m, RK, lmsbits, SA, lms_sorted);
+    extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
     if (m1) {
         u32 first = 0;
         d.read(&first, lms_text, sizeof(u32));
@@ -454,16 +484,16 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
 
     // ---- reduced problem ---------------------------------------------------
     u32 *SA1 = d.a->get<u32>(m1 + 1);
-    solve_reduced(rt, s, lms_sorted, lms_text, R, m1, SA1, depth, st);
+    solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st);
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
     labels_L(d, s, m, sbits, RK);
     // sorted LMS suffixes are grouped by first symbol already
-    if (m1) labels_lms_at_tails(d, s, SA1, m1, RK, cnt);
+    if (m1) labels_lms_at_tails(d, s, SA1, nullptr, m1, RK, cnt);
     chain_sort(rt, m, RK, Litems, nL, false);
     labels_S(d, s, m, sbits, RK);
     chain_sort(rt, m, RK, Sitems, nS, true);
-    write_sa_from_labels(d, m, RK, SA);
+    write_sa_from_labels(d, m, RK, SA, map);
     d.a->reset(mk);
 }
 

@@ -77,8 +77,8 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     }
 
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
-    u32 *R = d.a->get<u32>(n / 2 + 1);
-    bits_to_positions(d, lmsbits, nwords, lms_text, R);
+    u32 *NP = d.a->get<u32>(n / 2 + 1);
+    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
     u8 *dig = d.a->get<u8>(n);
 
     LevelStat st{n, m1, nL_total, 0, 0, 1};
@@ -105,17 +105,34 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     induce8(d, T, n, SA, B, tb, dig, dbits, SB, ENDS, K);
 
     // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
+    // (LMS flags of the SA entries packed once, then compacted)
     u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
-    scan_exclusive(
-        d, n, [=] __device__(u64 i) { return getbit(lmsbits, SA[i]); },
-        [=] __device__(u64 i, u32 off, u32 v) {
-            if (v) lms_sorted[off] = SA[i];
-        },
-        nullptr);
+    {
+        Arena::Mark m2 = d.a->mark();
+        u32 *fl = d.a->get<u32>(nwords + 1);
+        for_each_waves(d, n, [=] __device__(u64 i, bool live) {
+            u64 b = ballot(live && getbit(lmsbits, SA[i]));
+            const int l = lane_id();
+            if (l == 0 && i < n) fl[i >> 5] = (u32)b;
+            if (l == 32 && i < n) fl[i >> 5] = (u32)(b >> 32);
+        });
+        scan_exclusive(
+            d, nwords, [=] __device__(u64 w) { return (u32)__popc(fl[w]); },
+            [=] __device__(u64 w, u32 off, u32) {
+                u32 x = fl[w];
+                while (x) {
+                    int b = __ffs(x) - 1;
+                    x &= x - 1;
+                    lms_sorted[off++] = SA[w * 32 + b];
+                }
+            },
+            nullptr);
+        d.a->reset(m2);
+    }
 
     // ---- reduced problem (psacak8.rs:36-48) ---------------------------------
     u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
-    solve_reduced(rt, T, lms_sorted, lms_text, R, m1, SA1, 0, st, EB);
+    solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
     for_each(d, m1, [=] __device__(u64 j) {

@@ -78,6 +78,14 @@ inline void for_each(Dev &d, u64 n, F f) {
     PSK_LAUNCH_CHECK();
 }
 
+// Launch f(i, i < n) with one thread per item (wave-uniform control flow).
+template <typename F>
+inline void for_each_waves(Dev &d, u64 n, F f) {
+    if (n == 0) return;
+    k_for_each_waves<<<ceil_div(n, BLOCK), BLOCK, 0, d.s>>>(f, n);
+    PSK_LAUNCH_CHECK();
+}
+
 inline void Dev::fill_u32(u32 *dp, u32 v, u64 count) {
     for_each(*this, count, [=] __device__(u64 i) { dp[i] = v; });
 }
@@ -239,6 +247,19 @@ inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
 // stable order by digit and dstart the first position of its digit, so
 // gpos - dstart is its stable rank among items of that digit.
 // ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8), and each XCD has its own L2.  xcd_tile maps blocks so that each
+// XCD processes one contiguous range of tiles: tiles that write neighbouring
+// output (adjacent digit segments, adjacent count-matrix entries) then share
+// an L2, which merges their partial cache lines before they reach HBM.  The
+// map is a bijection of [0, nt).
+constexpr u32 NXCD = 8;
+__device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
+    u32 q = nt / NXCD, r = nt % NXCD;
+    u32 x = b % NXCD, i = b / NXCD;
+    return x * q + (x < r ? x : r) + i;
+}
+
 constexpr int MS_ROUNDS = 16;
 constexpr u64 MS_TILE = (u64)BLOCK * MS_ROUNDS;
 
@@ -247,7 +268,8 @@ __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
     __shared__ u32 h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    u64 base = (u64)blockIdx.x * MS_TILE;
+    const u32 t = xcd_tile(blockIdx.x, nt);
+    u64 base = (u64)t * MS_TILE;
     for (int r = 0; r < MS_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         u32 key = 0;
@@ -256,7 +278,7 @@ __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
         if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[key], (u32)__popcll(peers));
     }
     __syncthreads();
-    hist[(u64)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
+    hist[(u64)threadIdx.x * nt + t] = h[threadIdx.x];
 }
 
 template <class Dig, class Put>
@@ -264,12 +286,13 @@ __global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
     __shared__ u32 run[256];
     __shared__ u32 dst0[256];
     __shared__ u32 wc[NWAVES][256];
-    run[threadIdx.x] = off[(u64)threadIdx.x * nt + blockIdx.x];
+    const u32 t = xcd_tile(blockIdx.x, nt);
+    run[threadIdx.x] = off[(u64)threadIdx.x * nt + t];
     dst0[threadIdx.x] = off[(u64)threadIdx.x * nt];
     #pragma unroll
     for (int w = 0; w < NWAVES; w++) wc[w][threadIdx.x] = 0;
     __syncthreads();
-    u64 base = (u64)blockIdx.x * MS_TILE;
+    u64 base = (u64)t * MS_TILE;
     const int w = wave_id();
     for (int r = 0; r < MS_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
@@ -355,28 +378,37 @@ inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char
 constexpr int RS_ROUNDS = 8;
 constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
 
+// Counting: one block per RS_CT consecutive tiles (fewer, longer-lived
+// workgroups; each digit's RS_CT counts are written as one contiguous run).
+constexpr u32 RS_CT = 4;
 __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, u64 n, int shift,
                                                     u32 *hist, u32 nt) {
-    __shared__ u32 h[256];
-    h[threadIdx.x] = 0;
-    u64 base = (u64)blockIdx.x * RS_TILE;
-    u32 dk[RS_ROUNDS];
-    bool vv[RS_ROUNDS];
-    // all loads first (independent, in flight together)
+    __shared__ u32 h[RS_CT][256];
     #pragma unroll
-    for (int r = 0; r < RS_ROUNDS; r++) {
-        u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        vv[r] = i < n;
-        dk[r] = vv[r] ? (u32)(ki[i] >> shift) & 255u : 0u;
-    }
+    for (u32 c = 0; c < RS_CT; c++) h[c][threadIdx.x] = 0;
+    const u32 ng = (nt + RS_CT - 1) / RS_CT;
+    const u32 t0 = xcd_tile(blockIdx.x, ng) * RS_CT;
     __syncthreads();
+    // all loads of the block's tiles first (independent, in flight together)
+    u32 dk[RS_CT][RS_ROUNDS];
     #pragma unroll
-    for (int r = 0; r < RS_ROUNDS; r++) {
-        u64 peers = match_key8(vv[r], dk[r]);
-        if (vv[r] && (peers & lanemask_lt()) == 0) atomicAdd(&h[dk[r]], (u32)__popcll(peers));
-    }
+    for (u32 c = 0; c < RS_CT; c++)
+        #pragma unroll
+        for (int r = 0; r < RS_ROUNDS; r++) {
+            u64 i = (u64)(t0 + c) * RS_TILE + (u64)r * BLOCK + threadIdx.x;
+            dk[c][r] = i < n ? (u32)(ki[i] >> shift) & 255u : 256u;
+        }
+    #pragma unroll
+    for (u32 c = 0; c < RS_CT; c++)
+        #pragma unroll
+        for (int r = 0; r < RS_ROUNDS; r++) {
+            bool v = dk[c][r] < 256u;
+            u64 peers = match_key8(v, dk[c][r] & 255u);
+            if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[c][dk[c][r]], (u32)__popcll(peers));
+        }
     __syncthreads();
-    hist[(u64)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
+    for (u32 c = 0; c < RS_CT && t0 + c < nt; c++)
+        hist[(u64)threadIdx.x * nt + t0 + c] = h[c][threadIdx.x];
 }
 
 // off: exclusive scan of the digit-major count matrix (256*nt + 1 entries,
@@ -394,7 +426,7 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
     __shared__ u32 tstart[256], gbase[256];
     __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
     __shared__ u32 red[NWAVES + 1];
-    const u32 t = blockIdx.x;
+    const u32 t = xcd_tile(blockIdx.x, nt);
     const u32 dg = threadIdx.x;
     #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++)
@@ -476,7 +508,7 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
         u64 *ko = swapped ? k : kt;
         u32 *vo = swapped ? v : vt;
         int pc = d.pbegin("k_rs_count", 8.0 * n);
-        k_rs_count<<<nt, BLOCK, 0, d.s>>>(ki, n, shift, hist, nt);
+        k_rs_count<<<ceil_div(nt, RS_CT), BLOCK, 0, d.s>>>(ki, n, shift, hist, nt);
         PSK_LAUNCH_CHECK();
         d.pend(pc);
         scan_exclusive(

@@ -127,15 +127,73 @@ __device__ __forceinline__ u32 carry_from_right(bool has, u32 first, u32 block_c
     return higher ? f : wc;
 }
 
+// Mask sources for the aggregate / final kernels: word w -> (D, V).
+// From the text (byte texts, dword loads), or from mask words that
+// k_type_masks wrote (wider symbols).
 template <class C>
-__global__ void k_type_agg(const C *t, u64 n, u8 *blk_has, u8 *blk_first, const u32 *endbits) {
+struct MasksFromText {
+    const C *t;
+    u64 n;
+    const u32 *endbits;
+    __device__ void operator()(u64 w, u32 &D, u32 &V) const { type_masks(t, n, w * 32, D, V, endbits); }
+};
+struct MasksFromWords {
+    const u32 *Dw, *Vw;
+    u64 nwords;
+    __device__ void operator()(u64 w, u32 &D, u32 &V) const {
+        if (w < nwords) {
+            D = Dw[w];
+            V = Vw[w];
+        } else {
+            D = 0xFFFFFFFFu;  // padding positions are L-type
+            V = 0;
+        }
+    }
+};
+
+// D and V words of t with wave-wide loads: per step the 64 lanes read 64
+// consecutive symbols (coalesced) and two ballots give two words; lane l
+// keeps word l of its wave's 64.
+template <class C>
+__global__ void __launch_bounds__(BLOCK) k_type_masks(const C *t, u64 n, u64 nwords,
+                                                     const u32 *endbits, u32 *Dw, u32 *Vw) {
+    const int l = lane_id();
+    const u64 wbase = ((u64)blockIdx.x * NWAVES + wave_id()) * WAVE;
+    u32 myD = 0, myV = 0;
+    #pragma unroll 4
+    for (int s = 0; s < WAVE / 2; s++) {
+        u64 p = wbase * 32 + (u64)s * 64 + l;
+        u32 c0 = p < n ? (u32)t[p] : 0u;
+        u32 c1 = __shfl_down(c0, 1, WAVE);
+        if (l == WAVE - 1) c1 = p + 1 < n ? (u32)t[p + 1] : 0u;
+        bool last = p + 1 >= n;  // p == n-1 is L-type, p >= n padding (L)
+        u64 bd = ballot(last || c0 != c1);
+        u64 bv = ballot(!last && c0 < c1);
+        if (l == 2 * s) {
+            myD = (u32)bd;
+            myV = (u32)bv;
+        } else if (l == 2 * s + 1) {
+            myD = (u32)(bd >> 32);
+            myV = (u32)(bv >> 32);
+        }
+    }
+    u64 w = wbase + l;
+    if (w < nwords) {
+        apply_ends(endbits, w * 32, n, myD, myV);
+        Dw[w] = myD;
+        Vw[w] = myV;
+    }
+}
+
+template <class M>
+__global__ void k_type_agg(M masks, u8 *blk_has, u8 *blk_first) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
     u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
     bool has = false;
     u32 first = 0;
     for (int k = 0; k < TY_WPT; k++) {
         u32 D, V;
-        type_masks(t, n, (w0 + k) * 32, D, V, endbits);
+        masks(w0 + k, D, V);
         if (!has && D) {
             has = true;
             first = (V >> (__ffs(D) - 1)) & 1u;
@@ -200,9 +258,8 @@ __global__ void k_type_carry(const u8 *blk_has, const u8 *blk_first, u32 nb, u8 
     }
 }
 
-template <class C>
-__global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits, u64 nwords,
-                             const u32 *endbits, u32 *dbits) {
+template <class M>
+__global__ void k_type_final(M masks, const u8 *blk_carry, u32 *sbits, u64 nwords, u32 *dbits) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
     u64 w0 = ((u64)blockIdx.x * BLOCK + threadIdx.x) * TY_WPT;
     u32 D[TY_WPT], V[TY_WPT];
@@ -210,7 +267,7 @@ __global__ void k_type_final(const C *t, u64 n, const u8 *blk_carry, u32 *sbits,
     u32 first = 0;
     #pragma unroll
     for (int k = 0; k < TY_WPT; k++) {
-        type_masks(t, n, (w0 + k) * 32, D[k], V[k], endbits);
+        masks(w0 + k, D[k], V[k]);
         if (!has && D[k]) {
             has = true;
             first = (V[k] >> (__ffs(D[k]) - 1)) & 1u;
@@ -241,12 +298,22 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endb
     u8 *bh = d.a->get<u8>(nb);
     u8 *bf = d.a->get<u8>(nb);
     u8 *bc = d.a->get<u8>(nb);
-    k_type_agg<C><<<nb, BLOCK, 0, d.s>>>(t, n, bh, bf, endbits);
-    PSK_LAUNCH_CHECK();
-    k_type_carry<<<1, BLOCK, 0, d.s>>>(bh, bf, nb, bc);
-    PSK_LAUNCH_CHECK();
-    k_type_final<C><<<nb, BLOCK, 0, d.s>>>(t, n, bc, sbits, nwords, endbits, dbits);
-    PSK_LAUNCH_CHECK();
+    auto run = [&](auto masks) {
+        k_type_agg<<<nb, BLOCK, 0, d.s>>>(masks, bh, bf);
+        PSK_LAUNCH_CHECK();
+        k_type_carry<<<1, BLOCK, 0, d.s>>>(bh, bf, nb, bc);
+        PSK_LAUNCH_CHECK();
+        k_type_final<<<nb, BLOCK, 0, d.s>>>(masks, bc, sbits, nwords, dbits);
+        PSK_LAUNCH_CHECK();
+    };
+    if constexpr (sizeof(C) == 1) {
+        run(MasksFromText<C>{t, n, endbits});
+    } else {
+        u32 *Dw = d.a->get<u32>(nwords), *Vw = d.a->get<u32>(nwords);
+        k_type_masks<C><<<ceil_div(nwords, (u64)BLOCK), BLOCK, 0, d.s>>>(t, n, nwords, endbits, Dw, Vw);
+        PSK_LAUNCH_CHECK();
+        run(MasksFromWords{Dw, Vw, nwords});
+    }
     d.a->reset(mk);
 }
 
