@@ -218,8 +218,9 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32
                 atomicOr(err, 32u);
             } else {
                 u32 p = V[i];
-                RK[p] = lastconv ? (u32)hi - (g1sh[r * BLOCK + threadIdx.x] - s0)
-                                 : (u32)hi + (s0 - g0);
+                // the first subgroup of a group keeps its label hi: no write
+                if (lastconv) RK[p] = (u32)hi - (g1sh[r * BLOCK + threadIdx.x] - s0);
+                else if (s0 != g0) RK[p] = (u32)hi + (s0 - g0);
                 if (kp) nxt[kk - 1] = p;
             }
         }

@@ -380,7 +380,7 @@ constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
 
 // Counting: one block per RS_CT consecutive tiles (fewer, longer-lived
 // workgroups; each digit's RS_CT counts are written as one contiguous run).
-constexpr u32 RS_CT = 4;
+constexpr u32 RS_CT = 1;
 __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, u64 n, int shift,
                                                     u32 *hist, u32 nt) {
     __shared__ u32 h[RS_CT][256];
