@@ -191,11 +191,17 @@ struct Induce8Tables {
 
 // Full induce (L-scan then S-scan) over SA for a byte text, from the LMS
 // entries already placed at the bucket tails.  dig: scratch of >= n bytes.
+// PB: n bytes indexed by SA slot.  On entry, PB holds T[x-1] at the slot of
+// every placed LMS entry x (written by the placement, which reads T there
+// anyway); the L-scan reads those instead of gathering T, and stores at every
+// L slot the digit its entry will have in the S-scan (the predecessor byte if
+// it is smaller than the bucket's, else 255), so the S-scan reads its L parts
+// sequentially.  L slots are final after the L-scan, so the S-scan finds them.
 // Batches (SB != nullptr): K texts concatenated, ends[0..K) their last
 // positions; the L-scan starts from every text's sentinel, in text order.
 inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Induce8Tables tb,
-                    u8 *dig, const u32 *DB, const u32 *SB = nullptr, const u32 *ends = nullptr,
-                    u64 K = 1) {
+                    u8 *dig, u8 *PB, const u32 *DB, const u32 *SB = nullptr,
+                    const u32 *ends = nullptr, u64 K = 1) {
     const Pred pr{T, SB, DB};
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
     u32 *err = d.err;
@@ -228,15 +234,36 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         const u32 cc = (u32)c;
         if (nl) chain_queue8(d, pr, SA, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
         const u64 R = nl + nlms;
-        if (c == 255 || R == 0) continue;  // nothing can be induced above 255
+        if (c == 255) {
+            // nothing can be induced above 255; the S-scan digits of the L part
+            // (PB) still come from the cached predecessor bytes
+            if (nl) {
+                for_each(d, nl, [=] __device__(u64 j) {
+                    u32 x = SA[hc + j];
+                    u32 cp = dig[j];
+                    PB[hc + j] = (u8)((pr.has(x) && cp < cc) ? cp : 255u);
+                });
+            }
+            continue;
+        }
+        if (R == 0) continue;
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
         int ps = multisplit2(
             d, R,
             [=] __device__(u64 j, u32 *key) {
-                u32 x = SA[pos(j)];
-                // the queue part has its predecessor bytes cached
-                u32 cp = j < nl ? (u32)dig[j] : pr.byte(x);
-                u32 k = (pr.has(x) && cp > cc) ? cp : 0u;
+                u32 k;
+                if (j < nl) {
+                    // the queue part has its predecessor bytes cached in dig
+                    u32 x = SA[hc + j];
+                    u32 cp = dig[j];
+                    bool h = pr.has(x);
+                    k = (h && cp > cc) ? cp : 0u;
+                    PB[hc + j] = (u8)((h && cp < cc) ? cp : 255u);  // its S-scan digit
+                } else {
+                    // an LMS entry: its predecessor (L-type, so > c) is in PB
+                    u32 cp = PB[pos(j)];
+                    k = cp > cc ? cp : 0u;
+                }
                 dig[j] = (u8)k;  // 0 = not induced (valid digits are > c >= 0)
                 *key = k;
                 return k != 0;
@@ -272,9 +299,14 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         int ps = multisplit2(
             d, R,
             [=] __device__(u64 j, u32 *key) {
-                u32 x = SA[pos(j)];
-                u32 cp = j < ns ? (u32)dig[j] : pr.byte(x);
-                u32 k = (pr.has(x) && cp < cc) ? cp : 255u;
+                u32 k;
+                if (j < ns) {
+                    u32 x = SA[pos(j)];
+                    u32 cp = dig[j];
+                    k = (pr.has(x) && cp < cc) ? cp : 255u;
+                } else {
+                    k = PB[pos(j)];  // stored by the L-scan
+                }
                 dig[j] = (u8)k;  // 255 = not induced (valid digits are < c <= 255)
                 *key = k;
                 return k != 255;

@@ -91,10 +91,10 @@ struct RelIn {
 constexpr int REL_ROUNDS = 16;
 constexpr u64 REL_TILE = (u64)BLOCK * REL_ROUNDS;
 
-__global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1];
+__global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *tq) {
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
     u64 base = (u64)blockIdx.x * REL_TILE;
-    u32 g = 0, s = 0, k = 0, e = 0xFFFFFFFFu;
+    u32 g = 0, s = 0, k = 0, q = 0, e = 0xFFFFFFFFu;
     for (int r = 0; r < REL_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         if (i < in.A) {
@@ -103,13 +103,15 @@ __global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te) {
             if (gs) g = (u32)i;
             if (ss) s = (u32)i;
             k += kp;
+            q += kp && ss;
             if (ge && (u32)i < e) e = (u32)i;
         }
     }
-    u32 a1, a2, a3;
+    u32 a1, a2, a3, a4;
     block_incl<OpMax>(g, l1, &a1);
     block_incl<OpMax>(s, l2, &a2);
     block_incl<OpSum>(k, l3, &a3);
+    block_incl<OpSum>(q, l4, &a4);
     // min of group ends: minimum over the block
     u32 x = e;
     #pragma unroll
@@ -126,6 +128,7 @@ __global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te) {
         ts[blockIdx.x] = a2;
         tk[blockIdx.x] = a3;
         te[blockIdx.x] = m;
+        tq[blockIdx.x] = a4;
     }
 }
 
@@ -163,10 +166,16 @@ __global__ void k_rmin_tiles(u32 *a, u32 nt) {
     }
 }
 
+// Relabel and compact one round.  hi of a key is the item's group: its
+// label itself (glab == nullptr) or a dense group id whose label is glab[hi].
+// Kept items (not yet alone in their subgroup) go to nxt in sorted order, with
+// the dense id of their subgroup among the kept ones in gnx and that
+// subgroup's label in glnx[id].
 __global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32 *tg,
-                            const u32 *ts, const u32 *tk, const u32 *te, bool lastconv, u64 mm,
+                            const u32 *ts, const u32 *tk, const u32 *te, const u32 *tq,
+                            bool lastconv, u64 mm, const u32 *glab, u32 *gnx, u32 *glnx,
                             u32 *err) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1];
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
     __shared__ u32 wt[NWAVES];
     __shared__ u32 g1sh[REL_TILE];
     const u64 base = (u64)blockIdx.x * REL_TILE;
@@ -200,33 +209,42 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32
             carry = rm;
         }
     }
-    u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x];
+    u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x], rq = tq[blockIdx.x];
     for (int r = 0; r < REL_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         bool gs = false, ss = false, kp = false, ge = false;
         if (i < in.A) in.flags(i, gs, ss, kp, ge);
-        u32 a1, a2, a3;
+        u32 a1, a2, a3, a4;
         u32 g0 = block_incl<OpMax>(gs ? (u32)i : 0u, l1, &a1);
         u32 s0 = block_incl<OpMax>(ss ? (u32)i : 0u, l2, &a2);
         u32 kk = block_incl<OpSum>(kp ? 1u : 0u, l3, &a3);
+        u32 qq = block_incl<OpSum>((kp && ss) ? 1u : 0u, l4, &a4);
         g0 = g0 > rg ? g0 : rg;
         s0 = s0 > rs ? s0 : rs;
         kk += rk;
+        qq += rq;
         if (i < in.A) {
             u64 hi = in.K[i] >> in.B;
-            if (hi > mm) {
+            u64 lab = glab ? (u64)glab[hi] : hi;
+            if (lab > mm) {
                 atomicOr(err, 32u);
             } else {
                 u32 p = V[i];
-                // the first subgroup of a group keeps its label hi: no write
-                if (lastconv) RK[p] = (u32)hi - (g1sh[r * BLOCK + threadIdx.x] - s0);
-                else if (s0 != g0) RK[p] = (u32)hi + (s0 - g0);
-                if (kp) nxt[kk - 1] = p;
+                u32 nl = lastconv ? (u32)lab - (g1sh[r * BLOCK + threadIdx.x] - s0)
+                                  : (u32)lab + (s0 - g0);
+                // the first subgroup of a group keeps its label: no write
+                if (lastconv || s0 != g0) RK[p] = nl;
+                if (kp) {
+                    nxt[kk - 1] = p;
+                    gnx[kk - 1] = qq - 1;
+                    if (ss) glnx[qq - 1] = nl;
+                }
             }
         }
         rg = rg > a1 ? rg : a1;
         rs = rs > a2 ? rs : a2;
         rk += a3;
+        rq += a4;
     }
 }
 
@@ -235,24 +253,49 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32
 // If last_slot_labels, the items' initial group labels name the group's last
 // slot + 1 instead of its first (S parts are known by their tail); the first
 // round converts them, so later rounds use first-slot labels.
+// The high half of a key names the item's group.  Round 1 uses the label
+// itself, or, given rank (a dense rank of each position's symbol, k values),
+// the rank; every later round uses the dense id of the item's group among the
+// groups still being refined, with the groups' labels in a table.  Keys then
+// carry bit_width(groups) + bit_width(m) bits instead of 2 bit_width(m), and
+// the group is read with the item instead of gathered from RK.
+// tails (with rank): number of symbols of rank <= c, so the bucket of rank c
+// is [tails[c-1], tails[c]) and the initial labels are its head + 1 (L) or
+// its tail (S, last-slot labels).
 inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
-                       bool last_slot_labels) {
+                       bool last_slot_labels, const u32 *rank = nullptr,
+                       const u32 *tails = nullptr, u32 k = 0) {
     Dev &d = rt.d;
     if (A == 0) return;
     Arena::Mark mk = d.a->mark();
     u32 *act = d.a->get<u32>(A);
     u32 *vt = d.a->get<u32>(A);
+    u32 *gid = d.a->get<u32>(A), *gid2 = d.a->get<u32>(A);
+    const u64 gcap = std::max<u64>(rank ? k : 0, (u64)A / 2 + 1);
+    u32 *glab = d.a->get<u32>(gcap), *glab2 = d.a->get<u32>(gcap);
     u64 *keys = d.a->get<u64>(A);
     u64 *kt = d.a->get<u64>(A);
-    u32 *cnt = d.a->get<u32>(1);
+    u32 *cnt = d.a->get<u32>(2);
     PSK_HIP(hipMemcpyAsync(act, items, (u64)A * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
     const int B = bit_width(m);
+    int Bh = B;
+    const bool byrank = rank != nullptr && tails != nullptr && k > 0;
+    if (byrank) {
+        u32 *gl = glab;
+        const bool lsl = last_slot_labels;
+        for_each(d, k, [=] __device__(u64 c) {
+            gl[c] = lsl ? tails[c] : (c ? tails[c - 1] : 0u) + 1;
+        });
+        Bh = bit_width(k - 1);
+    }
     u64 h = 1;
     int rounds = 0;
     while (A > 0) {
         rounds++;
+        const bool first = rounds == 1;
         const u64 hh = h, mm = m;
         const u32 *ac = act;
+        const u32 *gi = gid;
         u32 *err = d.err;
         for_each(d, A, [=] __device__(u64 i) {
             u32 p = ac[i];
@@ -260,35 +303,45 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             u32 lo = 0;
             if (q <= mm) lo = RK[q];
             else atomicOr(err, 16u);
-            keys[i] = ((u64)RK[p] << B) | (u64)lo;
+            u32 hi = first ? (byrank ? rank[p] : RK[p]) : gi[i];
+            keys[i] = ((u64)hi << B) | (u64)lo;
         });
-        bool sw = radix_sort_kv(d, A, keys, act, kt, vt, 2 * B);
+        bool sw = radix_sort_kv(d, A, keys, act, kt, vt, Bh + B);
         const u64 *K = sw ? kt : keys;
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
         const u64 AA = A;
         {
-            const bool lastconv = last_slot_labels && rounds == 1;
+            const bool lastconv = last_slot_labels && first;
             Arena::Mark m3 = d.a->mark();
             const u32 nt = ceil_div(AA, REL_TILE);
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
+            u32 *tq = d.a->get<u32>(nt);
             RelIn in{K, AA, B};
-            k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te);
+            k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
             k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
             k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
             k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, cnt);
+            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tq, nt, cnt + 1);
             if (lastconv) k_rmin_tiles<<<1, BLOCK, 0, d.s>>>(te, nt);
             PSK_LAUNCH_CHECK();
+            const u32 *gl = (first && !byrank) ? nullptr : glab;
             int pr = d.pbegin("chain_relabel", 28.0 * AA);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, RK, nxt, tg, ts, tk, te, lastconv, mm, err);
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, RK, nxt, tg, ts, tk, te, tq, lastconv, mm,
+                                               gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
             d.a->reset(m3);
         }
-        A = d.read_u32(cnt);
+        u32 ag[2];
+        d.read(ag, cnt, sizeof(ag));
+        A = ag[0];
         if (nxt != act) std::swap(act, vt);
+        std::swap(gid, gid2);
+        std::swap(glab, glab2);
+        Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
         h *= 2;
         if (h > m + 1 && A > 0) fail(PSK_EDEVICE, "chain sort did not converge");
     }
@@ -344,7 +397,8 @@ inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64
 }
 
 void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
-             const u32 *map = nullptr);
+             const u32 *map = nullptr, const u32 *rank = nullptr, const u32 *tails = nullptr,
+             u32 k = 0);
 
 // SA of a level from complete labels RK (label = slot + 1); with map, the
 // entries are map[x] instead of x (the caller's positions of this level's
@@ -402,8 +456,10 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
     if (k < m1) {
         u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
         compute_types(d, (const u32 *)reduced, m1, sb);
-        translate_names(d, reduced, m1, sb, tails);
-        level32(rt, reduced, m1, sb, SA1, depth + 1, lms_text);
+        u32 *s1 = d.a->get<u32>(m1);
+        translate_names(d, reduced, s1, m1, sb, tails);
+        // the names stay as the dense symbol ranks of the next level
+        level32(rt, s1, m1, sb, SA1, depth + 1, lms_text, reduced, tails, k);
     } else {
         inverse_names(d, reduced, m1, SA1, lms_text);
     }
@@ -412,8 +468,10 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
 
 // One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
 // map (optional): SA entries are written as map[x] (see write_sa_from_labels).
+// rank, tails (optional): dense rank of every symbol of s (k distinct
+// values) and the cumulative symbol counts per rank.
 inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
-                    const u32 *map) {
+                    const u32 *map, const u32 *rank, const u32 *tails, u32 k) {
     Dev &d = rt.d;
     if (m == 0) return;
     if (m <= 3) {
@@ -471,9 +529,9 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         labels_lms_at_tails(d, s, sw ? v1 : v0, sw ? k1 : k0, m1, RK, cnt);
         d.a->reset(m2);
     }
-    chain_sort(rt, m, RK, Litems, nL, false);
+    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
     labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k);
     u32 *lms_sorted = d.a->get<u32>(m1 + 1);
     extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
     if (m1) {
@@ -491,9 +549,9 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     labels_L(d, s, m, sbits, RK);
     // sorted LMS suffixes are grouped by first symbol already
     if (m1) labels_lms_at_tails(d, s, SA1, nullptr, m1, RK, cnt);
-    chain_sort(rt, m, RK, Litems, nL, false);
+    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
     labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k);
     write_sa_from_labels(d, m, RK, SA, map);
     d.a->reset(mk);
 }

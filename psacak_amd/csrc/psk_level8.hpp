@@ -80,6 +80,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     u32 *NP = d.a->get<u32>(n / 2 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
     u8 *dig = d.a->get<u8>(n);
+    u8 *PB = d.a->get<u8>(n);
 
     LevelStat st{n, m1, nL_total, 0, 0, 1};
     rt.levels.resize(1);
@@ -99,10 +100,13 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             return true;
         },
         [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
-            SA[lmsbase[key] + (gpos - dstart)] = lms_text[i];
+            u32 p = lms_text[i];
+            u32 dst = lmsbase[key] + (gpos - dstart);
+            SA[dst] = p;
+            PB[dst] = T[p - 1];  // an LMS position is > 0 and never starts a text
         },
         nullptr);
-    induce8(d, T, n, SA, B, tb, dig, dbits, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
 
     // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
     // (LMS flags of the SA entries packed once, then compacted)
@@ -137,9 +141,11 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
     for_each(d, m1, [=] __device__(u64 j) {
         u32 p = SA1[j];
-        SA[j + lmsoff[T[p]]] = p;
+        u32 dst = j + lmsoff[T[p]];
+        SA[dst] = p;
+        PB[dst] = T[p - 1];
     });
-    induce8(d, T, n, SA, B, tb, dig, dbits, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
     d.a->reset(mk);
 }
 

@@ -142,10 +142,11 @@ inline u32 name_lms(Dev &d, const C *T, const u32 *lms_sorted, const u32 *lms_te
 
 // translate (naming.rs:263-274): L-type name c -> bucket head (tails[c-1] or
 // 0), S-type -> tails[c] - 1.  sbits are the types of the name string.
-inline void translate_names(Dev &d, u32 *reduced, u32 m1, const u32 *sbits, const u32 *tails) {
+inline void translate_names(Dev &d, const u32 *names, u32 *out, u32 m1, const u32 *sbits,
+                            const u32 *tails) {
     for_each(d, m1, [=] __device__(u64 i) {
-        u32 c = reduced[i];
-        reduced[i] = getbit(sbits, i) ? tails[c] - 1 : (c ? tails[c - 1] : 0u);
+        u32 c = names[i];
+        out[i] = getbit(sbits, i) ? tails[c] - 1 : (c ? tails[c - 1] : 0u);
     });
 }
 

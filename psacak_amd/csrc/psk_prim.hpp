@@ -274,8 +274,7 @@ __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         u32 key = 0;
         bool v = (i < n) && dig(i, &key);
-        u64 peers = match_key8(v, key);
-        if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[key], (u32)__popcll(peers));
+        if (v) atomicAdd(&h[key], 1u);
     }
     __syncthreads();
     hist[(u64)threadIdx.x * nt + t] = h[threadIdx.x];
@@ -398,14 +397,12 @@ __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, 
             u64 i = (u64)(t0 + c) * RS_TILE + (u64)r * BLOCK + threadIdx.x;
             dk[c][r] = i < n ? (u32)(ki[i] >> shift) & 255u : 256u;
         }
+    // plain LDS atomics: cheaper here than aggregating equal digits per wave
     #pragma unroll
     for (u32 c = 0; c < RS_CT; c++)
         #pragma unroll
-        for (int r = 0; r < RS_ROUNDS; r++) {
-            bool v = dk[c][r] < 256u;
-            u64 peers = match_key8(v, dk[c][r] & 255u);
-            if (v && (peers & lanemask_lt()) == 0) atomicAdd(&h[c][dk[c][r]], (u32)__popcll(peers));
-        }
+        for (int r = 0; r < RS_ROUNDS; r++)
+            if (dk[c][r] < 256u) atomicAdd(&h[c][dk[c][r]], 1u);
     __syncthreads();
     for (u32 c = 0; c < RS_CT && t0 + c < nt; c++)
         hist[(u64)threadIdx.x * nt + t0 + c] = h[c][threadIdx.x];

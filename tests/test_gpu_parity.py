@@ -92,6 +92,7 @@ ADVERSARIAL = {
     "runs": b"".join(bytes([65 + (i % 4)]) * (1 + (i * 7919) % 300) for i in range(2000)),
     "zero_runs": b"\x00" * 10000 + b"\x01" + b"\x00" * 9999,
     "ff00": b"\xff\x00" * 20000,
+    "ff_runs": (b"\xff\xff\xfe" * 9000 + b"\xff" * 700 + b"\x00\xff\xfd\xff") * 3,
 }
 
 
