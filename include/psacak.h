@@ -56,6 +56,13 @@ void psk_ctx_destroy(psk_ctx *ctx);
 const char *psk_last_error(const psk_ctx *ctx);
 /* Release cached device workspace (keeps the stream). */
 int psk_ctx_trim(psk_ctx *ctx);
+/* Naming strategy of this implementation (no counterpart in the crate, whose
+ * naming always follows induced sorting, psacak8.rs:57-72, naming.rs:20-259):
+ * LMS substrings are named by sorting them directly for at most `rounds`
+ * refinement rounds, and by induced sorting when ties remain after that.
+ * 0 = always induced sorting.  Default 6.  Results are identical either way.
+ * PSK_EINVAL if rounds is outside [0, 64]. */
+int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds);
 /* Device bytes a psacak8 call on a text of n bytes needs beyond its
  * input and output buffers (the admission footprint). */
 uint64_t psk_workspace_bytes8(uint64_t n);

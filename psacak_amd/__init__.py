@@ -66,6 +66,7 @@ def _load():
         "psk_ctx_destroy": ([vp], None),
         "psk_last_error": ([vp], ctypes.c_char_p),
         "psk_ctx_trim": ([vp], i32),
+        "psk_ctx_set_direct_rounds": ([vp, i32], i32),
         "psk_workspace_bytes8": ([u64], u64),
         "psk_workspace_bytes32": ([u64], u64),
         "psk_psacak_inplace": ([vp, vp, u64, vp, u64], i32),
@@ -172,6 +173,11 @@ class Context:
 
     def trim(self):
         self._check(_lib.psk_ctx_trim(self._h))
+
+    def set_direct_rounds(self, rounds: int):
+        """Rounds of direct LMS-substring sorting before induced sorting takes
+        over; 0 = always induced sorting.  Results are identical either way."""
+        self._check(_lib.psk_ctx_set_direct_rounds(self._h, int(rounds)))
 
     def last_levels(self):
         """Per-level rows (m, m1, mL, k, sum of LMS-substring lengths, symbol bytes)."""

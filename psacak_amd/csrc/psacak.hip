@@ -211,6 +211,12 @@ void psk_ctx_destroy(psk_ctx *ctx) {
 
 const char *psk_last_error(const psk_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
 
+int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds) {
+    if (!ctx || rounds < 0 || rounds > 64) return PSK_EINVAL;
+    ctx->rt.direct_rounds = rounds;
+    return PSK_OK;
+}
+
 int psk_ctx_trim(psk_ctx *ctx) {
     if (!ctx) return PSK_EINVAL;
     return guarded(ctx, [&] {

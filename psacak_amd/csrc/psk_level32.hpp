@@ -25,7 +25,7 @@
 // item) in the sorted order.
 #pragma once
 
-#include "psk_naming.hpp"
+#include "psk_lmsname.hpp"
 
 namespace psk {
 
@@ -37,6 +37,9 @@ struct Runtime {
     Dev d;
     std::vector<LevelStat> levels;
     int max_rounds = 0;
+    // rounds of direct LMS-substring sorting before induced sorting takes
+    // over (long repeated substrings); 0 = always induced sorting
+    int direct_rounds = 6;
 };
 
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.
@@ -60,192 +63,6 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
             }
         },
         nullptr);
-}
-
-// ---- fused relabel + compaction of one sorted round --------------------------
-// For sorted keys K (hi = old label, lo = successor label) and items V:
-//   g0(i) = first index of i's old group, g1(i) = its last index,
-//   s0(i) = first index of i's subgroup (items with an equal key),
-//   RK[V[i]] = hi + (s0 - g0)      (first-slot labels), or
-//              hi - (g1 - s0)      (last-slot labels),
-//   keep(i)  = i's subgroup has >= 2 items -> compacted, in order, into nxt.
-// One reduce pass (per tile: max of group starts, max of subgroup starts,
-// number kept, min of group ends) and one apply pass replace five passes
-// and an index table.
-struct RelIn {
-    const u64 *K;
-    u64 A;
-    int B;
-    // flags of item i: group start, subgroup start, kept, group end
-    __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
-        u64 k = K[i], hi = k >> B;
-        u64 kp_ = i ? K[i - 1] : ~0ull;
-        u64 kn_ = i + 1 < A ? K[i + 1] : ~0ull;
-        gs = i == 0 || (kp_ >> B) != hi;
-        ss = i == 0 || kp_ != k;
-        ge = i + 1 == A || (kn_ >> B) != hi;
-        kp = !ss || (i + 1 < A && kn_ == k);
-    }
-};
-
-constexpr int REL_ROUNDS = 16;
-constexpr u64 REL_TILE = (u64)BLOCK * REL_ROUNDS;
-
-__global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *tq) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
-    u64 base = (u64)blockIdx.x * REL_TILE;
-    u32 g = 0, s = 0, k = 0, q = 0, e = 0xFFFFFFFFu;
-    for (int r = 0; r < REL_ROUNDS; r++) {
-        u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        if (i < in.A) {
-            bool gs, ss, kp, ge;
-            in.flags(i, gs, ss, kp, ge);
-            if (gs) g = (u32)i;
-            if (ss) s = (u32)i;
-            k += kp;
-            q += kp && ss;
-            if (ge && (u32)i < e) e = (u32)i;
-        }
-    }
-    u32 a1, a2, a3, a4;
-    block_incl<OpMax>(g, l1, &a1);
-    block_incl<OpMax>(s, l2, &a2);
-    block_incl<OpSum>(k, l3, &a3);
-    block_incl<OpSum>(q, l4, &a4);
-    // min of group ends: minimum over the block
-    u32 x = e;
-    #pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) {
-        u32 y = __shfl_xor(x, o, WAVE);
-        x = x < y ? x : y;
-    }
-    if (lane_id() == 0) l1[wave_id()] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u32 m = 0xFFFFFFFFu;
-        for (int w = 0; w < NWAVES; w++) m = m < l1[w] ? m : l1[w];
-        tg[blockIdx.x] = a1;
-        ts[blockIdx.x] = a2;
-        tk[blockIdx.x] = a3;
-        te[blockIdx.x] = m;
-        tq[blockIdx.x] = a4;
-    }
-}
-
-// single workgroup: a[i] = min(a[i+1 ..]) (exclusive suffix minimum)
-__global__ void k_rmin_tiles(u32 *a, u32 nt) {
-    __shared__ u32 wt[NWAVES];
-    __shared__ u32 carry_sh;
-    const int l = lane_id(), w = wave_id();
-    if (threadIdx.x == 0) carry_sh = 0xFFFFFFFFu;
-    __syncthreads();
-    for (i64 base = (i64)((nt - 1) / BLOCK) * BLOCK; base >= 0; base -= BLOCK) {
-        u64 i = (u64)base + threadIdx.x;
-        u32 v = i < nt ? a[i] : 0xFFFFFFFFu;
-        u32 x = v;
-        #pragma unroll
-        for (int o = 1; o < WAVE; o <<= 1) {
-            u32 y = __shfl_down(x, o, WAVE);
-            if (l + o < WAVE) x = x < y ? x : y;
-        }
-        if (l == 0) wt[w] = x;
-        __syncthreads();
-        u32 carry = carry_sh;
-        u32 after = carry;
-        for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
-        u32 xn = __shfl_down(x, 1, WAVE);
-        u32 ex = (l + 1 < WAVE) ? (xn < after ? xn : after) : after;
-        if (i < nt) a[i] = ex;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            u32 m = carry;
-            for (int w2 = 0; w2 < NWAVES; w2++) m = m < wt[w2] ? m : wt[w2];
-            carry_sh = m;
-        }
-        __syncthreads();
-    }
-}
-
-// Relabel and compact one round.  hi of a key is the item's group: its
-// label itself (glab == nullptr) or a dense group id whose label is glab[hi].
-// Kept items (not yet alone in their subgroup) go to nxt in sorted order, with
-// the dense id of their subgroup among the kept ones in gnx and that
-// subgroup's label in glnx[id].
-__global__ void k_rel_apply(RelIn in, const u32 *V, u32 *RK, u32 *nxt, const u32 *tg,
-                            const u32 *ts, const u32 *tk, const u32 *te, const u32 *tq,
-                            bool lastconv, u64 mm, const u32 *glab, u32 *gnx, u32 *glnx,
-                            u32 *err) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
-    __shared__ u32 wt[NWAVES];
-    __shared__ u32 g1sh[REL_TILE];
-    const u64 base = (u64)blockIdx.x * REL_TILE;
-    const int l = lane_id(), w = wave_id();
-    if (lastconv) {
-        // g1 of every tile item: inclusive suffix minimum of group ends,
-        // rounds right to left, seeded with the tiles to the right
-        u32 carry = te[blockIdx.x];
-        for (int r = REL_ROUNDS - 1; r >= 0; r--) {
-            u64 i = base + (u64)r * BLOCK + threadIdx.x;
-            u32 e = 0xFFFFFFFFu;
-            if (i < in.A) {
-                bool gs, ss, kp, ge;
-                in.flags(i, gs, ss, kp, ge);
-                if (ge) e = (u32)i;
-            }
-            u32 x = e;
-            #pragma unroll
-            for (int o = 1; o < WAVE; o <<= 1) {
-                u32 y = __shfl_down(x, o, WAVE);
-                if (l + o < WAVE) x = x < y ? x : y;
-            }
-            if (l == 0) wt[w] = x;
-            __syncthreads();
-            u32 after = carry;
-            for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
-            g1sh[r * BLOCK + threadIdx.x] = x < after ? x : after;
-            u32 rm = carry;
-            for (int w2 = 0; w2 < NWAVES; w2++) rm = rm < wt[w2] ? rm : wt[w2];
-            __syncthreads();
-            carry = rm;
-        }
-    }
-    u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x], rq = tq[blockIdx.x];
-    for (int r = 0; r < REL_ROUNDS; r++) {
-        u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        bool gs = false, ss = false, kp = false, ge = false;
-        if (i < in.A) in.flags(i, gs, ss, kp, ge);
-        u32 a1, a2, a3, a4;
-        u32 g0 = block_incl<OpMax>(gs ? (u32)i : 0u, l1, &a1);
-        u32 s0 = block_incl<OpMax>(ss ? (u32)i : 0u, l2, &a2);
-        u32 kk = block_incl<OpSum>(kp ? 1u : 0u, l3, &a3);
-        u32 qq = block_incl<OpSum>((kp && ss) ? 1u : 0u, l4, &a4);
-        g0 = g0 > rg ? g0 : rg;
-        s0 = s0 > rs ? s0 : rs;
-        kk += rk;
-        qq += rq;
-        if (i < in.A) {
-            u64 hi = in.K[i] >> in.B;
-            u64 lab = glab ? (u64)glab[hi] : hi;
-            if (lab > mm) {
-                atomicOr(err, 32u);
-            } else {
-                u32 p = V[i];
-                u32 nl = lastconv ? (u32)lab - (g1sh[r * BLOCK + threadIdx.x] - s0)
-                                  : (u32)lab + (s0 - g0);
-                // the first subgroup of a group keeps its label: no write
-                if (lastconv || s0 != g0) RK[p] = nl;
-                if (kp) {
-                    nxt[kk - 1] = p;
-                    gnx[kk - 1] = qq - 1;
-                    if (ss) glnx[qq - 1] = nl;
-                }
-            }
-        }
-        rg = rg > a1 ? rg : a1;
-        rs = rs > a2 ? rs : a2;
-        rk += a3;
-        rq += a4;
-    }
 }
 
 // Sort items[0..A) by chain keys (see above); RK holds the labels of every
@@ -318,7 +135,7 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
             u32 *tq = d.a->get<u32>(nt);
-            RelIn in{K, AA, B};
+            RelIn in{K, AA, B, false};
             k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
             k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
@@ -329,8 +146,8 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             PSK_LAUNCH_CHECK();
             const u32 *gl = (first && !byrank) ? nullptr : glab;
             int pr = d.pbegin("chain_relabel", 28.0 * AA);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, RK, nxt, tg, ts, tk, te, tq, lastconv, mm,
-                                               gl, gid2, glab2, err);
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv}, nxt, tg, ts, tk, te,
+                                               tq, lastconv, mm, gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
             d.a->reset(m3);
@@ -435,7 +252,28 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lms_text
         nullptr);
 }
 
-// Solve the reduced problem of a level: names -> SA1 (as LMS positions).
+// The reduced problem of a level from its names (reduced, tails, k): types,
+// bucket-pointer translation and the next level, or the inverse when every
+// name is distinct.  SA1 receives the sorted LMS positions (lms_text maps a
+// reduced index to its position).
+inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u32 k,
+                            const u32 *lms_text, u32 m1, u32 *SA1, int depth) {
+    Dev &d = rt.d;
+    Arena::Mark mk = d.a->mark();
+    if (k < m1) {
+        u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
+        compute_types(d, reduced, m1, sb);
+        u32 *s1 = d.a->get<u32>(m1);
+        translate_names(d, reduced, s1, m1, sb, tails);
+        // the names stay as the dense symbol ranks of the next level
+        level32(rt, s1, m1, sb, SA1, depth + 1, lms_text, reduced, tails, k);
+    } else {
+        inverse_names(d, reduced, m1, SA1, lms_text);
+    }
+    d.a->reset(mk);
+}
+
+// Induced path: names from the sorted LMS substrings, then the reduced problem.
 template <class C>
 inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
                           const u32 *lmsbits, u64 nwords, u32 *NP, u32 m1, u32 *SA1, int depth,
@@ -451,19 +289,31 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
                      EB);
     st.k = k;
     rt.levels[depth] = st;
-    // SA1 of the reduced string, written as LMS positions (lms_text maps a
-    // reduced index to its position)
-    if (k < m1) {
-        u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
-        compute_types(d, (const u32 *)reduced, m1, sb);
-        u32 *s1 = d.a->get<u32>(m1);
-        translate_names(d, reduced, s1, m1, sb, tails);
-        // the names stay as the dense symbol ranks of the next level
-        level32(rt, s1, m1, sb, SA1, depth + 1, lms_text, reduced, tails, k);
-    } else {
-        inverse_names(d, reduced, m1, SA1, lms_text);
-    }
+    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth);
     d.a->reset(mk);
+}
+
+// Direct path: names by sorting the LMS substrings, then the reduced problem.
+// Returns false if the substrings need more rounds (nothing is written).
+template <class RF>
+inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32 *lms_text,
+                                 u32 m1, u32 *SA1, int depth, LevelStat &st) {
+    Dev &d = rt.d;
+    if (m1 == 0) return true;
+    if (rt.direct_rounds == 0) return false;
+    Arena::Mark mk = d.a->mark();
+    u32 *reduced = d.a->get<u32>(m1);
+    u32 *tails = d.a->get<u32>(m1);
+    u32 k = 0;
+    if (!name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds)) {
+        d.a->reset(mk);
+        return false;
+    }
+    st.k = k;
+    rt.levels[depth] = st;
+    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth);
+    d.a->reset(mk);
+    return true;
 }
 
 // One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
@@ -489,7 +339,6 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     PSK_LAUNCH_CHECK();
     const u32 m1 = d.read_u32(ctr);
     u32 *lms_text = d.a->get<u32>(m1 + 1);
-    u32 *NP = d.a->get<u32>(m / 2 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
 
     // L and S item lists (ascending positions)
@@ -513,37 +362,45 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     u32 *RK = d.a->get<u32>(m + 1);
     u32 *cnt = d.a->get<u32>(m + 1);
 
-    // ---- stage 1: sort LMS substrings --------------------------------------
-    // LMS characters at their bucket tails, in arbitrary order (psacak32.rs:60-99)
-    labels_L(d, s, m, sbits, RK);
-    if (m1) {
-        // LMS characters grouped by symbol (stable: text order inside a group)
-        Arena::Mark m2 = d.a->mark();
-        u64 *k0 = d.a->get<u64>(m1), *k1 = d.a->get<u64>(m1);
-        u32 *v0 = d.a->get<u32>(m1), *v1 = d.a->get<u32>(m1);
-        for_each(d, m1, [=] __device__(u64 r) {
-            k0[r] = s[lms_text[r]];
-            v0[r] = lms_text[r];
-        });
-        bool sw = radix_sort_kv(d, m1, k0, v0, k1, v1, bit_width(m));
-        labels_lms_at_tails(d, s, sw ? v1 : v0, sw ? k1 : k0, m1, RK, cnt);
-        d.a->reset(m2);
-    }
-    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
-    labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k);
-    u32 *lms_sorted = d.a->get<u32>(m1 + 1);
-    extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
+    // ---- LMS-substring names and the reduced problem ------------------------
     if (m1) {
         u32 first = 0;
         d.read(&first, lms_text, sizeof(u32));
         st.sumlen = m - first + m1;
     }
     rt.levels[depth] = st;
-
-    // ---- reduced problem ---------------------------------------------------
     u32 *SA1 = d.a->get<u32>(m1 + 1);
-    solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st);
+    // by sorting the substrings: symbols ranked by the names of the level
+    // above, else by their bucket-pointer values (order- and equality-
+    // preserving: types inside an LMS substring follow from its symbols)
+    const bool named =
+        rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st)
+             : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st);
+    if (!named) {
+        // induced stage 1 (long repeated LMS substrings): LMS characters at
+        // their bucket tails, in arbitrary order (psacak32.rs:60-99)
+        labels_L(d, s, m, sbits, RK);
+        {
+            // LMS characters grouped by symbol (stable: text order inside a group)
+            Arena::Mark m2 = d.a->mark();
+            u64 *k0 = d.a->get<u64>(m1), *k1 = d.a->get<u64>(m1);
+            u32 *v0 = d.a->get<u32>(m1), *v1 = d.a->get<u32>(m1);
+            for_each(d, m1, [=] __device__(u64 r) {
+                k0[r] = s[lms_text[r]];
+                v0[r] = lms_text[r];
+            });
+            bool sw = radix_sort_kv(d, m1, k0, v0, k1, v1, bit_width(m));
+            labels_lms_at_tails(d, s, sw ? v1 : v0, sw ? k1 : k0, m1, RK, cnt);
+            d.a->reset(m2);
+        }
+        chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
+        labels_S(d, s, m, sbits, RK);
+        chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k);
+        u32 *lms_sorted = d.a->get<u32>(m1 + 1);
+        extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
+        u32 *NP = d.a->get<u32>(m / 2 + 1);
+        solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st);
+    }
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
     labels_L(d, s, m, sbits, RK);

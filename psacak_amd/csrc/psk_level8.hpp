@@ -77,7 +77,6 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     }
 
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
-    u32 *NP = d.a->get<u32>(n / 2 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
     u8 *dig = d.a->get<u8>(n);
     u8 *PB = d.a->get<u8>(n);
@@ -91,52 +90,69 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     }
     rt.levels[0] = st;
 
-    // ---- stage 1: LMS characters to bucket tails (psacak8.rs:57-72), in text
-    // order within a bucket, then induce LMS substrings ----------------------
-    multisplit(
-        d, m1,
-        [=] __device__(u64 i, u32 *key) {
-            *key = T[lms_text[i]];
-            return true;
-        },
-        [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
-            u32 p = lms_text[i];
-            u32 dst = lmsbase[key] + (gpos - dstart);
-            SA[dst] = p;
-            PB[dst] = T[p - 1];  // an LMS position is > 0 and never starts a text
-        },
-        nullptr);
-    induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
-
-    // sorted LMS substrings = LMS entries in SA order (compact_left psacak8.rs:35)
-    // (LMS flags of the SA entries packed once, then compacted)
-    u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
-    {
-        Arena::Mark m2 = d.a->mark();
-        u32 *fl = d.a->get<u32>(nwords + 1);
-        for_each_waves(d, n, [=] __device__(u64 i, bool live) {
-            u64 b = ballot(live && getbit(lmsbits, SA[i]));
-            const int l = lane_id();
-            if (l == 0 && i < n) fl[i >> 5] = (u32)b;
-            if (l == 32 && i < n) fl[i >> 5] = (u32)(b >> 32);
-        });
-        scan_exclusive(
-            d, nwords, [=] __device__(u64 w) { return (u32)__popc(fl[w]); },
-            [=] __device__(u64 w, u32 off, u32) {
-                u32 x = fl[w];
-                while (x) {
-                    int b = __ffs(x) - 1;
-                    x &= x - 1;
-                    lms_sorted[off++] = SA[w * 32 + b];
-                }
+    // ---- LMS-substring names and the reduced problem (psacak8.rs:33-48) ------
+    u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
+    bool named = false;
+    if (!EB) {
+        // by sorting the substrings, bytes ranked among those present
+        u8 hcr[256];
+        u32 sigma = 0;
+        for (int c = 0; c < 256; c++) {
+            hcr[c] = (u8)sigma;
+            if (B.cnt[c]) sigma++;
+        }
+        u8 *crank = d.a->get<u8>(256);
+        d.write(crank, hcr, sizeof(hcr));
+        named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st);
+    }
+    if (!named) {
+        // induced stage 1 (batches, whose substrings that cross a text end must
+        // stay unique, and long repeated LMS substrings): LMS characters to
+        // bucket tails (psacak8.rs:57-72), in text order within a bucket, then
+        // induce the LMS substrings
+        multisplit(
+            d, m1,
+            [=] __device__(u64 i, u32 *key) {
+                *key = T[lms_text[i]];
+                return true;
+            },
+            [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
+                u32 p = lms_text[i];
+                u32 dst = lmsbase[key] + (gpos - dstart);
+                SA[dst] = p;
+                PB[dst] = T[p - 1];  // an LMS position is > 0 and never starts a text
             },
             nullptr);
-        d.a->reset(m2);
-    }
+        induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
 
-    // ---- reduced problem (psacak8.rs:36-48) ---------------------------------
-    u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
-    solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
+        // sorted LMS substrings = LMS entries in SA order (compact_left
+        // psacak8.rs:35): LMS flags of the SA entries packed once, then compacted
+        u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
+        {
+            Arena::Mark m2 = d.a->mark();
+            u32 *fl = d.a->get<u32>(nwords + 1);
+            for_each_waves(d, n, [=] __device__(u64 i, bool live) {
+                u64 b = ballot(live && getbit(lmsbits, SA[i]));
+                const int l = lane_id();
+                if (l == 0 && i < n) fl[i >> 5] = (u32)b;
+                if (l == 32 && i < n) fl[i >> 5] = (u32)(b >> 32);
+            });
+            scan_exclusive(
+                d, nwords, [=] __device__(u64 w) { return (u32)__popc(fl[w]); },
+                [=] __device__(u64 w, u32 off, u32) {
+                    u32 x = fl[w];
+                    while (x) {
+                        int b = __ffs(x) - 1;
+                        x &= x - 1;
+                        lms_sorted[off++] = SA[w * 32 + b];
+                    }
+                },
+                nullptr);
+            d.a->reset(m2);
+        }
+        u32 *NP = d.a->get<u32>(n / 2 + 1);
+        solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
+    }
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
     for_each(d, m1, [=] __device__(u64 j) {

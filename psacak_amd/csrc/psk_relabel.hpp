@@ -1,0 +1,219 @@
+// psk_relabel.hpp -- relabel and compaction of one sorted refinement round,
+// shared by the chain sort of the integer levels and the direct sort of LMS
+// substrings.
+//
+// For sorted keys K (hi = the item's group, lo = what refines it) and items V:
+//   g0(i) = first index of i's group, g1(i) = its last index,
+//   s0(i) = first index of i's subgroup (items with an equal key),
+//   label(i) = base + (s0 - g0)    (first-slot labels), or
+//              base - (g1 - s0)    (last-slot labels),
+//   where base is the group's label (hi itself, or glab[hi] for dense ids),
+//   keep(i)  = i's subgroup has >= 2 items (and, with a terminated bit, is
+//              not terminated) -> compacted, in order, into nxt.
+// One reduce pass (per tile: max of group starts, max of subgroup starts,
+// number kept, min of group ends, number of kept subgroups) and one apply pass.
+#pragma once
+
+#include "psk_naming.hpp"
+
+namespace psk {
+
+struct RelIn {
+    const u64 *K;
+    u64 A;
+    int B;
+    bool tbit;  // bit 0 of a key set: the item is finished (never kept)
+    // flags of item i: group start, subgroup start, kept, group end
+    __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
+        u64 k = K[i], hi = k >> B;
+        u64 kp_ = i ? K[i - 1] : ~0ull;
+        u64 kn_ = i + 1 < A ? K[i + 1] : ~0ull;
+        gs = i == 0 || (kp_ >> B) != hi;
+        ss = i == 0 || kp_ != k;
+        ge = i + 1 == A || (kn_ >> B) != hi;
+        kp = (!ss || (i + 1 < A && kn_ == k)) && !(tbit && (k & 1u));
+    }
+};
+
+// What a round does with an item's new label.  ChainSink: positions' labels
+// (the first subgroup of a group keeps its label, so it is not rewritten).
+// FinishSink: an item that is not kept records its subgroup's slot, and the
+// subgroup's first item marks that slot as a group start.
+struct ChainSink {
+    u32 *RK;
+    bool lastconv;
+    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool first_sub, bool, bool) const {
+        if (lastconv || !first_sub) RK[p] = nl;
+    }
+};
+struct FinishSink {
+    u32 *L, *F;
+    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool, bool kept, bool sstart) const {
+        if (!kept) {
+            L[p] = nl;
+            if (sstart) F[nl] = 1;
+        }
+    }
+};
+
+constexpr int REL_ROUNDS = 16;
+constexpr u64 REL_TILE = (u64)BLOCK * REL_ROUNDS;
+
+__global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *tq) {
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
+    u64 base = (u64)blockIdx.x * REL_TILE;
+    u32 g = 0, s = 0, k = 0, q = 0, e = 0xFFFFFFFFu;
+    for (int r = 0; r < REL_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        if (i < in.A) {
+            bool gs, ss, kp, ge;
+            in.flags(i, gs, ss, kp, ge);
+            if (gs) g = (u32)i;
+            if (ss) s = (u32)i;
+            k += kp;
+            q += kp && ss;
+            if (ge && (u32)i < e) e = (u32)i;
+        }
+    }
+    u32 a1, a2, a3, a4;
+    block_incl<OpMax>(g, l1, &a1);
+    block_incl<OpMax>(s, l2, &a2);
+    block_incl<OpSum>(k, l3, &a3);
+    block_incl<OpSum>(q, l4, &a4);
+    // min of group ends: minimum over the block
+    u32 x = e;
+    #pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        u32 y = __shfl_xor(x, o, WAVE);
+        x = x < y ? x : y;
+    }
+    if (lane_id() == 0) l1[wave_id()] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 m = 0xFFFFFFFFu;
+        for (int w = 0; w < NWAVES; w++) m = m < l1[w] ? m : l1[w];
+        tg[blockIdx.x] = a1;
+        ts[blockIdx.x] = a2;
+        tk[blockIdx.x] = a3;
+        te[blockIdx.x] = m;
+        tq[blockIdx.x] = a4;
+    }
+}
+
+// single workgroup: a[i] = min(a[i+1 ..]) (exclusive suffix minimum)
+__global__ void k_rmin_tiles(u32 *a, u32 nt) {
+    __shared__ u32 wt[NWAVES];
+    __shared__ u32 carry_sh;
+    const int l = lane_id(), w = wave_id();
+    if (threadIdx.x == 0) carry_sh = 0xFFFFFFFFu;
+    __syncthreads();
+    for (i64 base = (i64)((nt - 1) / BLOCK) * BLOCK; base >= 0; base -= BLOCK) {
+        u64 i = (u64)base + threadIdx.x;
+        u32 v = i < nt ? a[i] : 0xFFFFFFFFu;
+        u32 x = v;
+        #pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            u32 y = __shfl_down(x, o, WAVE);
+            if (l + o < WAVE) x = x < y ? x : y;
+        }
+        if (l == 0) wt[w] = x;
+        __syncthreads();
+        u32 carry = carry_sh;
+        u32 after = carry;
+        for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
+        u32 xn = __shfl_down(x, 1, WAVE);
+        u32 ex = (l + 1 < WAVE) ? (xn < after ? xn : after) : after;
+        if (i < nt) a[i] = ex;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 m = carry;
+            for (int w2 = 0; w2 < NWAVES; w2++) m = m < wt[w2] ? m : wt[w2];
+            carry_sh = m;
+        }
+        __syncthreads();
+    }
+}
+
+// Relabel and compact one round.  hi of a key is the item's group: its
+// label itself (glab == nullptr) or a dense group id whose label is glab[hi].
+// Kept items (not yet alone in their subgroup) go to nxt in sorted order, with
+// the dense id of their subgroup among the kept ones in gnx and that
+// subgroup's label in glnx[id].
+template <class Sink>
+__global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u32 *tg,
+                            const u32 *ts, const u32 *tk, const u32 *te, const u32 *tq,
+                            bool lastconv, u64 mm, const u32 *glab, u32 *gnx, u32 *glnx,
+                            u32 *err) {
+    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
+    __shared__ u32 wt[NWAVES];
+    __shared__ u32 g1sh[REL_TILE];
+    const u64 base = (u64)blockIdx.x * REL_TILE;
+    const int l = lane_id(), w = wave_id();
+    if (lastconv) {
+        // g1 of every tile item: inclusive suffix minimum of group ends,
+        // rounds right to left, seeded with the tiles to the right
+        u32 carry = te[blockIdx.x];
+        for (int r = REL_ROUNDS - 1; r >= 0; r--) {
+            u64 i = base + (u64)r * BLOCK + threadIdx.x;
+            u32 e = 0xFFFFFFFFu;
+            if (i < in.A) {
+                bool gs, ss, kp, ge;
+                in.flags(i, gs, ss, kp, ge);
+                if (ge) e = (u32)i;
+            }
+            u32 x = e;
+            #pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                u32 y = __shfl_down(x, o, WAVE);
+                if (l + o < WAVE) x = x < y ? x : y;
+            }
+            if (l == 0) wt[w] = x;
+            __syncthreads();
+            u32 after = carry;
+            for (int w2 = NWAVES - 1; w2 > w; w2--) after = after < wt[w2] ? after : wt[w2];
+            g1sh[r * BLOCK + threadIdx.x] = x < after ? x : after;
+            u32 rm = carry;
+            for (int w2 = 0; w2 < NWAVES; w2++) rm = rm < wt[w2] ? rm : wt[w2];
+            __syncthreads();
+            carry = rm;
+        }
+    }
+    u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x], rq = tq[blockIdx.x];
+    for (int r = 0; r < REL_ROUNDS; r++) {
+        u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        bool gs = false, ss = false, kp = false, ge = false;
+        if (i < in.A) in.flags(i, gs, ss, kp, ge);
+        u32 a1, a2, a3, a4;
+        u32 g0 = block_incl<OpMax>(gs ? (u32)i : 0u, l1, &a1);
+        u32 s0 = block_incl<OpMax>(ss ? (u32)i : 0u, l2, &a2);
+        u32 kk = block_incl<OpSum>(kp ? 1u : 0u, l3, &a3);
+        u32 qq = block_incl<OpSum>((kp && ss) ? 1u : 0u, l4, &a4);
+        g0 = g0 > rg ? g0 : rg;
+        s0 = s0 > rs ? s0 : rs;
+        kk += rk;
+        qq += rq;
+        if (i < in.A) {
+            u64 hi = in.K[i] >> in.B;
+            u64 lab = glab ? (u64)glab[hi] : hi;
+            if (lab > mm) {
+                atomicOr(err, 32u);
+            } else {
+                u32 p = V[i];
+                u32 nl = lastconv ? (u32)lab - (g1sh[r * BLOCK + threadIdx.x] - s0)
+                                  : (u32)lab + (s0 - g0);
+                sink(p, nl, s0 == g0, kp, ss);
+                if (kp) {
+                    nxt[kk - 1] = p;
+                    gnx[kk - 1] = qq - 1;
+                    if (ss) glnx[qq - 1] = nl;
+                }
+            }
+        }
+        rg = rg > a1 ? rg : a1;
+        rs = rs > a2 ? rs : a2;
+        rk += a3;
+        rq += a4;
+    }
+}
+
+}  // namespace psk

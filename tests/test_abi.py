@@ -85,6 +85,16 @@ def test_context_create_without_device():
         assert c.last_levels().shape[1] == 6
 
 
+def test_direct_rounds_setting_without_gpu():
+    with P.Context() as c:
+        for r in (0, 1, 6, 64):
+            c.set_direct_rounds(r)
+        for r in (-1, 65):
+            with pytest.raises(P.PskError) as e:
+                c.set_direct_rounds(r)
+            assert e.value.code == P.PSK_EINVAL
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
 def test_no_device_is_an_error_not_an_abort():
     with pytest.raises(P.PskError) as e:

@@ -132,6 +132,33 @@ def test_uniform_alphabets(ctx, alphabet):
     assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
 
 
+@pytest.mark.parametrize("rounds", [0, 1, 2, 6])
+def test_naming_strategies_agree(rounds):
+    # direct LMS-substring sorting with at most `rounds` rounds, induced
+    # sorting where ties remain (0 = induced only), at every level
+    note(f"naming rounds={rounds}")
+    rng = np.random.default_rng(31)
+    texts = [gen.markov_dna(1 << 20, seed=9), ADVERSARIAL["descending"], ADVERSARIAL["runs"],
+             gen.uniform8(200000, alphabet=3, seed=2), b"abcab" * 20000 + b"x" + b"abcab" * 9000]
+    a, b = b"a", b"ab"
+    while len(b) < 50000:
+        a, b = b, b + a
+    texts.append(b)
+    with P.Context() as c:
+        c.set_direct_rounds(rounds)
+        for i, t in enumerate(texts):
+            note(f"naming rounds={rounds} text {i}")
+            assert np.array_equal(P.psacak(t, c), oracle.sacak8(t)), i
+        for it in range(40):
+            n = int(rng.integers(2, 400))
+            u = rng.integers(0, int(rng.choice([2, 3, 10, 50])), n).astype(np.uint32)
+            tr = oracle.translate32(u)
+            if tr is not None:
+                assert np.array_equal(P.psacak32(tr, c), oracle.sacak32(tr)), it
+        z = gen.zipf32(1 << 18, sigma=1 << 12, s=1.1, repeat_frac=0.3, seed=3)
+        assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
+
+
 def test_psacak32_random_bucket_pointer_texts(ctx):
     rng = np.random.default_rng(99)
     for it in range(150):
