@@ -60,7 +60,8 @@ int psk_ctx_trim(psk_ctx *ctx);
  * naming always follows induced sorting, psacak8.rs:57-72, naming.rs:20-259):
  * LMS substrings are named by sorting them directly for at most `rounds`
  * refinement rounds, and by induced sorting when ties remain after that.
- * 0 = always induced sorting.  Default 6.  Results are identical either way.
+ * (or earlier when a quarter of them are still tied after 4 rounds).
+ * 0 = always induced sorting.  Default 32.  Results are identical either way.
  * PSK_EINVAL if rounds is outside [0, 64]. */
 int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds);
 /* Device bytes a psacak8 call on a text of n bytes needs beyond its

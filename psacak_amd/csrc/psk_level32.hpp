@@ -39,7 +39,7 @@ struct Runtime {
     int max_rounds = 0;
     // rounds of direct LMS-substring sorting before induced sorting takes
     // over (long repeated substrings); 0 = always induced sorting
-    int direct_rounds = 6;
+    int direct_rounds = 32;
 };
 
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.

@@ -92,10 +92,15 @@ inline int lms_first_digits(const u32 *hist, u32 m1, int Db) {
     return bestD;
 }
 
+// Rounds after which a quarter of the substrings still tied means long
+// repeated substrings (induced sorting is then the cheaper way).
+constexpr int LMS_TIED_CHECK_ROUND = 4;
+
 // Names of the m1 LMS substrings of a text of length m (LMS positions
 // lms_text, ascending): reduced[r] = name of the r-th, tails[c] = number of
 // substrings with name <= c, *k = number of names.  Returns false (outputs
-// undefined) if ties remain after max_rounds.
+// undefined) if ties remain after max_rounds, or if more than a quarter of
+// the substrings are still tied after LMS_TIED_CHECK_ROUND rounds.
 template <class RF>
 inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
                             u32 *reduced, u32 *tails, u32 *k, int max_rounds) {
@@ -132,7 +137,7 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
     const u64 term = sigma + 1, mm = m;
     const u32 mm1 = m1;
     while (A > 0) {
-        if (rounds == max_rounds) {
+        if (rounds == max_rounds || (rounds >= LMS_TIED_CHECK_ROUND && A > m1 / 4)) {
             d.a->reset(mk);
             return false;
         }
