@@ -17,6 +17,31 @@ struct BatchInfo {
 // SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
 // batch of texts, each with its own sentinel, and SA receives the SA of the
 // concatenation in which every text's suffixes are in their own order.
+// Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87):
+// entry j of SA1 starts with the byte c whose LMS range [lstart[c],
+// lstart[c+1]) holds j (binary search in LDS) and goes to j + lmsoff[c]; the
+// byte before it is cached in PB for the L-scan.
+__global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
+                                                     const u32 *lmsoff, const u8 *T, u32 *SA,
+                                                     u8 *PB) {
+    __shared__ u32 st[257], off[256];
+    for (int i = threadIdx.x; i < 257; i += BLOCK) st[i] = lstart[i];
+    off[threadIdx.x] = lmsoff[threadIdx.x];
+    __syncthreads();
+    for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < m1; j += (u64)gridDim.x * BLOCK) {
+        u32 p = SA1[j];
+        u32 lo = 0, hi = 256;  // largest c with st[c] <= j; st[256] = m1 > j
+        while (hi - lo > 1) {
+            u32 mid = (lo + hi) >> 1;
+            if (st[mid] <= (u32)j) lo = mid;
+            else hi = mid;
+        }
+        u32 dst = (u32)j + off[lo];
+        SA[dst] = p;
+        PB[dst] = T[p - 1];  // an LMS position is > 0
+    }
+}
+
 inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInfo *bi = nullptr) {
     Dev &d = rt.d;
     rt.levels.clear();
@@ -155,12 +180,22 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     }
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
-    for_each(d, m1, [=] __device__(u64 j) {
-        u32 p = SA1[j];
-        u32 dst = j + lmsoff[T[p]];
-        SA[dst] = p;
-        PB[dst] = T[p - 1];
-    });
+    // SA1 is sorted, so its entries come grouped by first byte: the bucket of
+    // entry j follows from the LMS counts, without reading T[p]
+    if (m1) {
+        u32 *lstart = d.a->get<u32>(257);
+        u32 h[257];
+        u64 acc = 0;
+        for (int c = 0; c < 256; c++) {
+            h[c] = (u32)acc;
+            acc += B.nLMS[c];
+        }
+        h[256] = (u32)acc;
+        d.write(lstart, h, sizeof(h));
+        u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 65536);
+        k_place_lms<<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, T, SA, PB);
+        PSK_LAUNCH_CHECK();
+    }
     induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
     d.a->reset(mk);
 }
