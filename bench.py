@@ -445,7 +445,11 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if dom and os.path.exists(tpath):
         tj = json.load(open(tpath))
-        if tj.get("kernel") == dom["name"] and args.config == "markov1g":
+        # only a measurement of this kernel at this launch size applies
+        alg_now = dom["bytes"] / max(dom["launches"], 1)
+        alg_then = float(tj.get("algorithmic_bytes_per_launch", 0))
+        if (tj.get("kernel") == dom["name"] and args.config == "markov1g"
+                and abs(alg_then - alg_now) <= 0.01 * alg_now):
             traffic = int(tj["traffic_bytes_per_launch"])
     roof = None
     if dom and dom["launches"]:

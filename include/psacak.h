@@ -64,6 +64,13 @@ int psk_ctx_trim(psk_ctx *ctx);
  * 0 = always induced sorting.  Default 32.  Results are identical either way.
  * PSK_EINVAL if rounds is outside [0, 64]. */
 int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds);
+/* Round 1 of that direct naming on a byte text (level 0): 1 (default) groups
+ * the LMS substrings through a hash dictionary of their distinct leading
+ * symbols and sorts only the distinct keys; 0 sorts every substring's key.
+ * A text whose substrings take too many distinct values uses sorting either
+ * way.  No counterpart in the crate; results are identical.  PSK_EINVAL
+ * unless on is 0 or 1. */
+int psk_ctx_set_dict_naming(psk_ctx *ctx, int on);
 /* Device bytes a psacak8 call on a text of n bytes needs beyond its
  * input and output buffers (the admission footprint). */
 uint64_t psk_workspace_bytes8(uint64_t n);

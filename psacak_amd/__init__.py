@@ -67,6 +67,7 @@ def _load():
         "psk_last_error": ([vp], ctypes.c_char_p),
         "psk_ctx_trim": ([vp], i32),
         "psk_ctx_set_direct_rounds": ([vp, i32], i32),
+        "psk_ctx_set_dict_naming": ([vp, i32], i32),
         "psk_workspace_bytes8": ([u64], u64),
         "psk_workspace_bytes32": ([u64], u64),
         "psk_psacak_inplace": ([vp, vp, u64, vp, u64], i32),
@@ -178,6 +179,12 @@ class Context:
         """Rounds of direct LMS-substring sorting before induced sorting takes
         over; 0 = always induced sorting.  Results are identical either way."""
         self._check(_lib.psk_ctx_set_direct_rounds(self._h, int(rounds)))
+
+    def set_dict_naming(self, on: bool):
+        """Round 1 of direct naming at level 0 through a dictionary of the
+        distinct LMS-substring keys (True, default) or by sorting every key
+        (False).  Results are identical either way."""
+        self._check(_lib.psk_ctx_set_dict_naming(self._h, 1 if on else 0))
 
     def last_levels(self):
         """Per-level rows (m, m1, mL, k, sum of LMS-substring lengths, symbol bytes)."""

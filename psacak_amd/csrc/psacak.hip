@@ -217,6 +217,12 @@ int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds) {
     return PSK_OK;
 }
 
+int psk_ctx_set_dict_naming(psk_ctx *ctx, int on) {
+    if (!ctx || on < 0 || on > 1) return PSK_EINVAL;
+    ctx->rt.dict_naming = on != 0;
+    return PSK_OK;
+}
+
 int psk_ctx_trim(psk_ctx *ctx) {
     if (!ctx) return PSK_EINVAL;
     return guarded(ctx, [&] {
@@ -241,7 +247,8 @@ uint64_t psk_workspace_bytes32(uint64_t n) {
 }
 
 uint64_t psk_workspace_bytes8(uint64_t n) {
-    return n / 4 + n + 10 * n + psk_workspace_bytes32(n / 2) + (64ull << 20);
+    // bit vectors, digit cache, byte windows (8n), LMS lists and names
+    return n / 4 + n + 17 * n + psk_workspace_bytes32(n / 2) + (64ull << 20);
 }
 
 int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa,

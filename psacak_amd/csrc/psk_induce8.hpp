@@ -22,6 +22,16 @@
 // Every read position is final before it is read, so no empty marks are
 // needed and the S parts are rebuilt from scratch (the LMS entries placed for
 // the L-scan are overwritten, as in the reference).
+//
+// Windows.  Inducing x-1 from x needs T[x-1]; the suffix induced next from
+// x-1 needs T[x-2], and so on down the text, a random gather each.  Every SA
+// slot therefore carries a window (PW, one u64 per slot) of the bytes before
+// its entry: T[x-1], T[x-2], ... (up to 7).  A placed LMS suffix gets its
+// window from one gather; an induced entry x-1 inherits the window of x
+// shifted by one byte.  The chain of inductions from an LMS position down to
+// the previous one spans one LMS substring, so a window covers it whenever
+// that substring is at most 7 bytes long (97% of them for DNA); a window that
+// has run out is refilled from the text when it is needed.
 #pragma once
 
 #include "psk_types.hpp"
@@ -31,6 +41,26 @@ namespace psk {
 struct Buckets8 {
     u32 cnt[256], nL[256], nLMS[256];
     u64 head[257];
+};
+
+// Window of the bytes before an entry x: bits 0..55 hold T[x-1], T[x-2], ...
+// (byte i at bits 8(i-1)), bits 56..58 the number v of valid bytes, bit 59
+// is set when position x-v has no predecessor (the text, or in a batch the
+// entry's own text, starts there).  v = 0 without bit 59: not known yet.
+struct Win {
+    static constexpr u64 BYTES = (1ull << 56) - 1;
+    __device__ __forceinline__ static u32 v(u64 w) { return (u32)(w >> 56) & 7u; }
+    __device__ __forceinline__ static bool e(u64 w) { return (w >> 59) & 1ull; }
+    __device__ __forceinline__ static bool known(u64 w) { return (w >> 56) != 0; }
+    // byte i >= 1 (T[x-i]); i <= v
+    __device__ __forceinline__ static u32 b(u64 w, u32 i) { return (u32)(w >> (8 * (i - 1))) & 255u; }
+    __device__ __forceinline__ static u64 make(u64 bytes, u32 v, bool e) {
+        return (bytes & BYTES) | ((u64)v << 56) | ((u64)e << 59);
+    }
+    // window of x - t from the window of x (t <= v)
+    __device__ __forceinline__ static u64 shift(u64 w, u32 t) {
+        return make((w & BYTES) >> (8 * t), v(w) - t, e(w));
+    }
 };
 
 // queue position of read index j: base + dir * j
@@ -64,12 +94,82 @@ struct Pred {
         u32 top = w * 32 + (31 - __clz(m));
         return q - top;
     }
+    // Window of x gathered from the text: up to 7 bytes, stopping where a
+    // position has no predecessor.  With ALIGNED (T 8-byte aligned) the bytes
+    // come from the (at most two) aligned u64 words that hold them.
+    template <bool ALIGNED>
+    __device__ __forceinline__ u64 window(u32 x) const {
+        u32 v = x < 7u ? x : 7u;  // bytes before x in the array
+        bool e = x <= 7u;
+        if (SB && v) {
+            // a text start s in [x-v+1, x] leaves the bytes T[s..x-1]
+            u32 lo = x - v + 1;
+            u32 wa = lo >> 5, wb = x >> 5;
+            u32 mb = SB[wb] & (0xFFFFFFFFu >> (31 - (x & 31)));
+            if (wb == wa) mb &= 0xFFFFFFFFu << (lo & 31);
+            u32 s = 0;
+            bool found = false;
+            if (mb) {
+                s = wb * 32 + (31 - __clz(mb));
+                found = true;
+            } else if (wa != wb) {
+                u32 ma = SB[wa] & (0xFFFFFFFFu << (lo & 31));
+                if (ma) {
+                    s = wa * 32 + (31 - __clz(ma));
+                    found = true;
+                }
+            }
+            if (found) {
+                v = x - s;
+                e = true;
+            }
+        }
+        u64 bytes = 0;
+        if (v) {
+            if (ALIGNED) {
+                const u64 *Q = (const u64 *)T;
+                u32 lo = x - v;
+                u32 wa = lo >> 3, wb = (x - 1) >> 3;
+                u64 q0 = Q[wa];
+                u64 q1 = wb != wa ? Q[wb] : 0ull;
+                #pragma unroll
+                for (u32 i = 1; i <= 7; i++) {
+                    if (i <= v) {
+                        u32 pos = x - i;
+                        u64 q = (pos >> 3) == wa ? q0 : q1;
+                        bytes |= ((q >> (8 * (pos & 7))) & 255ull) << (8 * (i - 1));
+                    }
+                }
+            } else {
+                #pragma unroll
+                for (u32 i = 1; i <= 7; i++)
+                    if (i <= v) bytes |= (u64)T[x - i] << (8 * (i - 1));
+            }
+        }
+        return Win::make(bytes, v, e);
+    }
+    __device__ __forceinline__ u64 window_any(u32 x, bool aligned) const {
+        return aligned ? window<true>(x) : window<false>(x);
+    }
 };
 
-// Every queue entry x gets its predecessor byte cached in dig[j] (j = read
-// index), gathered once: the chain step needs "T[x-1] == c" and the
-// multi-split that follows needs T[x-1] as its digit.  A suffix without a
-// predecessor caches 0; consumers test Pred::has themselves.
+// Window of the entry x - t (1 <= t <= r) of a chain: seed x whose window is
+// w, followed by its run of r bytes c; sb = T[x-r-1] when x - r has a
+// predecessor (has_sb).
+__device__ __forceinline__ u64 chain_window(u64 w, u32 r, u32 c, bool has_sb, u32 sb, u32 t) {
+    const u32 v = Win::v(w);
+    if (r < v) return Win::shift(w, t);
+    // the run covers the window: T[x-t-1 .. x-r] = c, then sb or the text start
+    const u32 k = r - t;
+    const u32 vv = k < 7u ? k : 7u;
+    u64 bytes = (u64)c * 0x0101010101010101ull;
+    bytes &= vv ? (~0ull >> (64 - 8 * vv)) : 0ull;
+    if (vv < 7u) {
+        if (has_sb) return Win::make(bytes | ((u64)sb << (8 * vv)), vv + 1, false);
+        return Win::make(bytes, vv, true);
+    }
+    return Win::make(bytes, 7, false);
+}
 
 // Fill one bucket's queue: positions qp(0..limit); generation 0 (the entries
 // induced from other buckets) is [0, *d_start).  Every further entry is a
@@ -80,9 +180,10 @@ struct Pred {
 // r_k' >= t}, i.e. the entries emitted seed by seed, stably sorted by depth.
 // One pass measures the runs, one scan gives emission offsets, and one stable
 // multi-split by depth places them (two passes if a run exceeds 256).
-// Caches every entry's predecessor byte in dig[0..limit).  d_tmp: 2 u32.
-inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
-                         const u32 *d_start, u64 limit, u32 *d_tmp) {
+// Leaves every queue entry's window known (refilled where it had run out)
+// and its predecessor byte (0 if none) in dig[0..limit).  d_tmp: 2 u32.
+inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *dig, QPos qp,
+                         u32 c, const u32 *d_start, u64 limit, u32 *d_tmp) {
     const u64 a = d.read_u32(d_start);
     u32 *err = d.err;
     if (a > limit) fail(PSK_EDEVICE, "queue seed count exceeds its bucket part");
@@ -93,7 +194,7 @@ inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
     d.zero(maxr, sizeof(u32));
     const u64 nchain = limit - a;
     {
-        int prec = d.pbegin("chain8_runs", 10.0 * a);
+        int prec = d.pbegin("chain8_runs", 22.0 * a);
         const u64 aa = a;
         // one thread per seed; the grid is a multiple of the wave so every
         // lane takes part in the wave-wide maximum (one atomic per wave)
@@ -102,13 +203,27 @@ inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
             [=] __device__(u64 k, bool live) {
                 u32 r = 0;
                 if (live) {
-                    u32 x = SA[qp(k)];
-                    u32 cp = pr.byte(x);
+                    const u64 slot = qp(k);
+                    u32 x = SA[slot];
+                    u64 w = PW[slot];
+                    if (!Win::known(w)) {
+                        w = pr.window_any(x, aligned);
+                        PW[slot] = w;
+                    }
+                    const u32 v = Win::v(w);
+                    const u32 cp = v ? Win::b(w, 1) : 0u;
                     dig[k] = (u8)cp;
-                    u32 sb = cp;
-                    if (pr.has(x) && cp == c) {
-                        r = pr.run_before(x);  // T[x] == c: the run of c's before x
-                        sb = pr.byte(x - r);   // predecessor byte of the chain's last entry
+                    u32 sb = 0;
+                    if (v && cp == c) {
+                        // the run of c's before x: inside the window, or
+                        // beyond it from the decided bits
+                        r = 1;
+                        while (r < v && Win::b(w, r + 1) == c) r++;
+                        if (r < v) sb = Win::b(w, r + 1);
+                        else if (!Win::e(w)) {
+                            r = pr.run_before(x);
+                            sb = pr.byte(x - r);
+                        }
                     }
                     run[k] = r;
                     stop[k] = (u8)sb;
@@ -134,17 +249,23 @@ inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
     d.read(hv, d_tmp, sizeof(hv));
     const u32 R = hv[0];
     if ((u64)hv[1] != nchain) fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
-    // emit (depth-1, entry, its predecessor byte) seed by seed
+    // emit (depth-1, entry, its predecessor byte, its window) seed by seed
     u32 *ek = d.a->get<u32>(nchain), *ev = d.a->get<u32>(nchain);
     u8 *eb = d.a->get<u8>(nchain);
+    u64 *ew = d.a->get<u64>(nchain);
     for_each(d, a, [=] __device__(u64 k) {
         u32 r = run[k];
         if (!r) return;
-        u32 x = SA[qp(k)], o = eo[k];
+        const u64 slot = qp(k);
+        u32 x = SA[slot], o = eo[k];
+        u64 w = PW[slot];
+        u32 sb = stop[k];
+        bool hs = pr.has(x - r);
         for (u32 t = 1; t <= r; t++) {
             ek[o + t - 1] = t - 1;
             ev[o + t - 1] = x - t;
-            eb[o + t - 1] = t < r ? (u8)c : stop[k];
+            eb[o + t - 1] = t < r ? (u8)c : (u8)sb;
+            ew[o + t - 1] = chain_window(w, r, c, hs, sb, t);
         }
     });
     const u64 lim = limit, base = a;
@@ -158,7 +279,9 @@ inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
             [=] __device__(u64 j, u32, u32 gpos, u32) {
                 u64 q = base + gpos;
                 if (q < lim) {
-                    SA[qp(q)] = ev[j];
+                    const u64 slot = qp(q);
+                    SA[slot] = ev[j];
+                    PW[slot] = ew[j];
                     dig[q] = eb[j];
                 } else {
                     atomicOr(err, 2u);
@@ -177,7 +300,9 @@ inline void chain_queue8(Dev &d, Pred pr, u32 *SA, u8 *dig, QPos qp, u32 c,
         const u32 *V = sw ? v1 : v0;
         for_each(d, nchain, [=] __device__(u64 i) {
             u32 j = V[i];
-            SA[qp(base + i)] = ev[j];
+            const u64 slot = qp(base + i);
+            SA[slot] = ev[j];
+            PW[slot] = ew[j];
             dig[base + i] = eb[j];
         });
     }
@@ -190,19 +315,15 @@ struct Induce8Tables {
 };
 
 // Full induce (L-scan then S-scan) over SA for a byte text, from the LMS
-// entries already placed at the bucket tails.  dig: scratch of >= n bytes.
-// PB: n bytes indexed by SA slot.  On entry, PB holds T[x-1] at the slot of
-// every placed LMS entry x (written by the placement, which reads T there
-// anyway); the L-scan reads those instead of gathering T, and stores at every
-// L slot the digit its entry will have in the S-scan (the predecessor byte if
-// it is smaller than the bucket's, else 255), so the S-scan reads its L parts
-// sequentially.  L slots are final after the L-scan, so the S-scan finds them.
+// entries already placed at the bucket tails with their windows in PW (one
+// u64 per SA slot).  dig: scratch of >= n bytes.
 // Batches (SB != nullptr): K texts concatenated, ends[0..K) their last
 // positions; the L-scan starts from every text's sentinel, in text order.
 inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Induce8Tables tb,
-                    u8 *dig, u8 *PB, const u32 *DB, const u32 *SB = nullptr,
+                    u8 *dig, u64 *PW, const u32 *DB, const u32 *SB = nullptr,
                     const u32 *ends = nullptr, u64 K = 1) {
     const Pred pr{T, SB, DB};
+    const bool aligned = (((uintptr_t)T) & 7u) == 0;
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
     u32 *err = d.err;
     const u64 nn = n;
@@ -212,6 +333,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         for_each(d, 1, [=] __device__(u64) {
             u32 c = T[nn - 1];
             SA[head[c]] = (u32)(nn - 1);  // the sentinel induces n-1 first
+            PW[head[c]] = pr.window<false>((u32)(nn - 1));
             fill[c] = 1;
         });
     } else {
@@ -223,7 +345,9 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                 return true;
             },
             [=] __device__(u64 i, u32 key, u32 gpos, u32 dstart) {
-                SA[head[key] + (gpos - dstart)] = ends[i];
+                u32 slot = head[key] + (gpos - dstart);
+                SA[slot] = ends[i];
+                PW[slot] = pr.window<false>(ends[i]);
             },
             counts);
         for_each(d, 256, [=] __device__(u64 k) { fill[k] = counts[k]; });
@@ -232,38 +356,18 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], nlms = B.nLMS[c];
         const u32 cc = (u32)c;
-        if (nl) chain_queue8(d, pr, SA, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
+        if (nl) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
         const u64 R = nl + nlms;
-        if (c == 255) {
-            // nothing can be induced above 255; the S-scan digits of the L part
-            // (PB) still come from the cached predecessor bytes
-            if (nl) {
-                for_each(d, nl, [=] __device__(u64 j) {
-                    u32 x = SA[hc + j];
-                    u32 cp = dig[j];
-                    PB[hc + j] = (u8)((pr.has(x) && cp < cc) ? cp : 255u);
-                });
-            }
-            continue;
-        }
-        if (R == 0) continue;
+        if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
         int ps = multisplit2(
             d, R,
             [=] __device__(u64 j, u32 *key) {
-                u32 k;
-                if (j < nl) {
-                    // the queue part has its predecessor bytes cached in dig
-                    u32 x = SA[hc + j];
-                    u32 cp = dig[j];
-                    bool h = pr.has(x);
-                    k = (h && cp > cc) ? cp : 0u;
-                    PB[hc + j] = (u8)((h && cp < cc) ? cp : 255u);  // its S-scan digit
-                } else {
-                    // an LMS entry: its predecessor (L-type, so > c) is in PB
-                    u32 cp = PB[pos(j)];
-                    k = cp > cc ? cp : 0u;
-                }
+                // predecessor byte: cached for the queue part; from the window
+                // for an LMS entry (which always has a predecessor).  No
+                // predecessor reads 0, which is never > c.
+                u32 cp = j < nl ? (u32)dig[j] : Win::b(PW[pos(j)], 1);
+                u32 k = cp > cc ? cp : 0u;
                 dig[j] = (u8)k;  // 0 = not induced (valid digits are > c >= 0)
                 *key = k;
                 return k != 0;
@@ -274,16 +378,22 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                 return k != 0;
             },
             [=] __device__(u64 j, u32 key, u32 gpos, u32 dstart) {
-                u32 x = SA[pos(j)];
+                const u64 src = pos(j);
+                u32 x = SA[src];
+                u64 w = PW[src];
                 u64 dst = (u64)head[key] + fill[key] + (gpos - dstart);
-                if (dst < nn) SA[dst] = x - 1;
-                else atomicOr(err, 4u);
+                if (dst < nn) {
+                    SA[dst] = x - 1;
+                    PW[dst] = Win::shift(w, 1);
+                } else {
+                    atomicOr(err, 4u);
+                }
             },
-            counts, "induce8", 6.0 * R, 5.0 * R);
+            counts, "induce8", 10.0 * R, 5.0 * R);
         u64 *slot = d.pslot(ps);
         for_each(d, 256, [=] __device__(u64 k) {
             fill[k] += counts[k];
-            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 4ull * counts[k]);
+            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 20ull * counts[k]);
         });
     }
     // ---------------- S-scan ----------------
@@ -292,7 +402,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
         const u32 cc = (u32)c;
-        if (ns) chain_queue8(d, pr, SA, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
+        if (ns) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };
@@ -305,7 +415,9 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                     u32 cp = dig[j];
                     k = (pr.has(x) && cp < cc) ? cp : 255u;
                 } else {
-                    k = PB[pos(j)];  // stored by the L-scan
+                    // an L entry: its window is known since its bucket's L-scan
+                    u64 w = PW[pos(j)];
+                    k = (Win::v(w) && Win::b(w, 1) < cc) ? Win::b(w, 1) : 255u;
                 }
                 dig[j] = (u8)k;  // 255 = not induced (valid digits are < c <= 255)
                 *key = k;
@@ -317,16 +429,22 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
                 return k != 255;
             },
             [=] __device__(u64 j, u32 key, u32 gpos, u32 dstart) {
-                u32 x = SA[pos(j)];
+                const u64 src = pos(j);
+                u32 x = SA[src];
+                u64 w = PW[src];
                 i64 dst = (i64)tail[key] - 1 - (i64)(fill[key] + (gpos - dstart));
-                if (dst >= 0 && dst < (i64)nn) SA[dst] = x - 1;
-                else atomicOr(err, 8u);
+                if (dst >= 0 && dst < (i64)nn) {
+                    SA[dst] = x - 1;
+                    PW[dst] = Win::shift(w, 1);
+                } else {
+                    atomicOr(err, 8u);
+                }
             },
             counts, "induce8", 6.0 * R, 5.0 * R);
         u64 *slot = d.pslot(ps);
         for_each(d, 256, [=] __device__(u64 k) {
             fill[k] += counts[k];
-            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 4ull * counts[k]);
+            if (slot && counts[k]) atomicAdd((unsigned long long *)slot, 20ull * counts[k]);
         });
     }
 }

@@ -40,6 +40,9 @@ struct Runtime {
     // rounds of direct LMS-substring sorting before induced sorting takes
     // over (long repeated substrings); 0 = always induced sorting
     int direct_rounds = 32;
+    // round 1 of direct naming at level 0 through a dictionary of the
+    // distinct keys (false: by sorting every key)
+    bool dict_naming = true;
 };
 
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.
@@ -305,7 +308,14 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     u32 *reduced = d.a->get<u32>(m1);
     u32 *tails = d.a->get<u32>(m1);
     u32 k = 0;
-    if (!name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds)) {
+    // a byte text's LMS substrings take few distinct values: try the
+    // dictionary first there
+    bool dict = false;
+    if constexpr (std::is_same<RF, ByteRanks>::value)
+        dict = rt.dict_naming &&
+               name_lms_dict(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds);
+    if (!dict &&
+        !name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds)) {
         d.a->reset(mk);
         return false;
     }

@@ -14,16 +14,14 @@ struct BatchInfo {
     const u32 *endbits, *startbits;
 };
 
-// SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
-// batch of texts, each with its own sentinel, and SA receives the SA of the
-// concatenation in which every text's suffixes are in their own order.
 // Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87):
 // entry j of SA1 starts with the byte c whose LMS range [lstart[c],
-// lstart[c+1]) holds j (binary search in LDS) and goes to j + lmsoff[c]; the
-// byte before it is cached in PB for the L-scan.
+// lstart[c+1]) holds j (binary search in LDS) and goes to j + lmsoff[c],
+// with the window of the bytes before it (PW) for the induce.
+template <bool ALIGNED>
 __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
-                                                     const u32 *lmsoff, const u8 *T, u32 *SA,
-                                                     u8 *PB) {
+                                                     const u32 *lmsoff, Pred pr, u32 *SA,
+                                                     u64 *PW) {
     __shared__ u32 st[257], off[256];
     for (int i = threadIdx.x; i < 257; i += BLOCK) st[i] = lstart[i];
     off[threadIdx.x] = lmsoff[threadIdx.x];
@@ -38,10 +36,13 @@ __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, con
         }
         u32 dst = (u32)j + off[lo];
         SA[dst] = p;
-        PB[dst] = T[p - 1];  // an LMS position is > 0
+        PW[dst] = pr.window<ALIGNED>(p);
     }
 }
 
+// SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
+// batch of texts, each with its own sentinel, and SA receives the SA of the
+// concatenation in which every text's suffixes are in their own order.
 inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInfo *bi = nullptr) {
     Dev &d = rt.d;
     rt.levels.clear();
@@ -104,7 +105,8 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
     u8 *dig = d.a->get<u8>(n);
-    u8 *PB = d.a->get<u8>(n);
+    u64 *PW = d.a->get<u64>(n);
+    const Pred pr{T, SB, dbits};
 
     LevelStat st{n, m1, nL_total, 0, 0, 1};
     rt.levels.resize(1);
@@ -145,10 +147,10 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
                 u32 p = lms_text[i];
                 u32 dst = lmsbase[key] + (gpos - dstart);
                 SA[dst] = p;
-                PB[dst] = T[p - 1];  // an LMS position is > 0 and never starts a text
+                PW[dst] = pr.window<false>(p);
             },
             nullptr);
-        induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
+        induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);
 
         // sorted LMS substrings = LMS entries in SA order (compact_left
         // psacak8.rs:35): LMS flags of the SA entries packed once, then compacted
@@ -193,10 +195,13 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         h[256] = (u32)acc;
         d.write(lstart, h, sizeof(h));
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 65536);
-        k_place_lms<<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, T, SA, PB);
+        if ((((uintptr_t)T) & 7u) == 0)
+            k_place_lms<true><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, pr, SA, PW);
+        else
+            k_place_lms<false><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, pr, SA, PW);
         PSK_LAUNCH_CHECK();
     }
-    induce8(d, T, n, SA, B, tb, dig, PB, dbits, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);
     d.a->reset(mk);
 }
 

@@ -22,13 +22,17 @@
 // the same first symbol.  Equal digit strings up to the terminator are equal
 // substrings.
 //
-// Rounds: each sorts the tied, unfinished substrings by (dense id of their
-// tie group, next D digits, finished bit) with the LSD radix sort and splits
-// the groups; a substring is finished once its terminator has been compared or
+// Round 1 puts the substrings into groups by their first D digits, either by
+// sorting all m1 keys (name_lms_direct) or, when the keys take few distinct
+// values, through a hash dictionary of the distinct keys (name_lms_dict):
+// only the distinct keys are sorted, and the substrings are counted per key.
+// Later rounds sort the tied, unfinished substrings by (dense id of their tie
+// group, next D digits, finished bit) with the LSD radix sort and split the
+// groups; a substring is finished once its terminator has been compared or
 // it is alone in its group.  Finished substrings record the slot of their
-// group in the final order; a scan over the group-start marks turns slots into
-// names.  Long repeated substrings need many rounds: after max_rounds the
-// caller falls back to induced sorting.
+// group in the final order; a scan over the group-start marks turns slots
+// into names.  Long repeated substrings need many rounds: after max_rounds
+// the caller falls back to induced sorting.
 #pragma once
 
 #include "psk_relabel.hpp"
@@ -96,6 +100,137 @@ inline int lms_first_digits(const u32 *hist, u32 m1, int Db) {
 // repeated substrings (induced sorting is then the cheaper way).
 constexpr int LMS_TIED_CHECK_ROUND = 4;
 
+// Key of the r-th LMS substring for a round that compares digits [o, o+D):
+// the digits packed Db bits each (first digit highest), then the finished
+// bit (the terminator, or the sentinel digit of the last substring, lies
+// within these digits).  D*Db + 1 <= 63, so a key never has bit 63 set.
+template <class RF>
+__device__ __forceinline__ u64 lms_key(RF rk, const u32 *lms_text, u32 m1, u64 m, u32 r, u64 o,
+                                       int D, int Db, u64 term) {
+    u64 p = lms_text[r];
+    bool last = r + 1 == m1;
+    u64 len = last ? m - p : (u64)lms_text[r + 1] - p + 1;
+    u64 td = last ? 0ull : term;
+    u64 acc = 0;
+    for (int j = 0; j < D; j++) {
+        u64 idx = o + (u64)j;
+        u64 dg = idx < len ? (u64)rk(p + idx) + 1 : (idx == len ? td : 0ull);
+        acc = (acc << Db) | dg;
+    }
+    return (acc << 1) | (len < o + (u64)D ? 1ull : 0ull);
+}
+
+// Working state of the refinement rounds.  act[0..A): the substrings (their
+// text-order index r) still tied; gid[i]: dense id of item i's tie group,
+// whose label (first slot) is glab[gid]; L[r]: slot of a finished substring;
+// F[slot] = 1 at the first slot of every finished group.
+struct LmsRounds {
+    u32 *act, *vt, *gid, *gid2, *glab, *glab2;
+    u64 *keys, *kt;
+    u32 *L, *F, *cnt;
+    u32 A = 0;
+    u64 off = 0;  // digits compared so far by every tied substring
+    int Bg = 0;   // bits of the group ids
+    int rounds = 0;
+
+    void alloc(Dev &d, u32 m1) {
+        act = d.a->get<u32>(m1);
+        vt = d.a->get<u32>(m1);
+        gid = d.a->get<u32>(m1);
+        gid2 = d.a->get<u32>(m1);
+        glab = d.a->get<u32>((u64)m1 / 2 + 1);
+        glab2 = d.a->get<u32>((u64)m1 / 2 + 1);
+        keys = d.a->get<u64>(m1);
+        kt = d.a->get<u64>(m1);
+        L = d.a->get<u32>(m1);
+        F = d.a->get<u32>(m1);
+        cnt = d.a->get<u32>(2);
+    }
+};
+
+// Rounds until no substring is tied.  Round 1 (rounds == 0 on entry) sorts
+// every substring by its first D1 digits; later rounds use the group ids.
+// Returns false when the round limits are reached with ties left.
+template <class RF>
+inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32 *lms_text,
+                       u32 m1, int D1, int max_rounds) {
+    const int Db = bit_width(sigma + 1);
+    const u64 term = sigma + 1, mm = m;
+    const u32 mm1 = m1;
+    u32 *err = d.err;
+    while (st.A > 0) {
+        if (st.rounds >= max_rounds || (st.rounds >= LMS_TIED_CHECK_ROUND && st.A > m1 / 4))
+            return false;
+        st.rounds++;
+        const bool first = st.rounds == 1;
+        const int D = first ? D1 : (62 - st.Bg) / Db;
+        const int LB = D * Db + 1;  // digits + finished bit; keys keep <= 63 bits
+        const u64 o = st.off;
+        const int DD = D, DB = Db;
+        const u32 *ac = st.act, *gi = st.gid;
+        u64 *keys = st.keys;
+        for_each(d, st.A, [=] __device__(u64 i) {
+            u32 r = ac[i];
+            u64 hi = first ? 0ull : (u64)gi[i];
+            keys[i] = (hi << LB) | lms_key(rk, lms_text, mm1, mm, r, o, DD, DB, term);
+        });
+        bool sw = radix_sort_kv(d, st.A, st.keys, st.act, st.kt, st.vt, st.Bg + LB);
+        const u64 *K = sw ? st.kt : st.keys;
+        const u32 *V = sw ? st.vt : st.act;
+        u32 *nxt = sw ? st.act : st.vt;
+        {
+            Arena::Mark m3 = d.a->mark();
+            const u32 nt = ceil_div(st.A, REL_TILE);
+            u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
+            u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
+            u32 *tq = d.a->get<u32>(nt);
+            RelIn in{K, st.A, LB, true};
+            k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
+            PSK_LAUNCH_CHECK();
+            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
+            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
+            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, st.cnt);
+            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tq, nt, st.cnt + 1);
+            PSK_LAUNCH_CHECK();
+            int pr = d.pbegin("lms_name_relabel", 28.0 * st.A);
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, FinishSink{st.L, st.F}, nxt, tg, ts, tk, te,
+                                               tq, false, (u64)m1, first ? nullptr : st.glab,
+                                               st.gid2, st.glab2, err);
+            PSK_LAUNCH_CHECK();
+            d.pend(pr);
+            d.a->reset(m3);
+        }
+        u32 ag[2];
+        d.read(ag, st.cnt, sizeof(ag));
+        st.A = ag[0];
+        if (nxt != st.act) std::swap(st.act, st.vt);
+        std::swap(st.gid, st.gid2);
+        std::swap(st.glab, st.glab2);
+        st.Bg = bit_width(ag[1] ? ag[1] - 1 : 0);
+        st.off += (u64)D;
+    }
+    return true;
+}
+
+// Slots -> names once nothing is tied: the marked group starts, counted in
+// slot order.  tails[c] = number of substrings with name <= c; returns k.
+inline u32 lms_slot_names(Dev &d, LmsRounds &st, u32 m1, u32 *tails) {
+    u32 *tl = tails, *F = st.F;
+    const u32 mm1 = m1;
+    scan_exclusive(
+        d, m1, [=] __device__(u64 j) { return F[j]; },
+        [=] __device__(u64 j, u32 ex, u32 v) {
+            if (v) {
+                F[j] = ex;
+                if (ex) tl[ex - 1] = (u32)j;
+            }
+        },
+        st.cnt);
+    const u32 kk = d.read_u32(st.cnt);
+    for_each(d, 1, [=] __device__(u64) { tl[kk - 1] = mm1; });
+    return kk;
+}
+
 // Names of the m1 LMS substrings of a text of length m (LMS positions
 // lms_text, ascending): reduced[r] = name of the r-th, tails[c] = number of
 // substrings with name <= c, *k = number of names.  Returns false (outputs
@@ -110,108 +245,366 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
     }
     Arena::Mark mk = d.a->mark();
     const int Db = bit_width(sigma + 1);
-    u32 *cnt = d.a->get<u32>(LEN_BINS + 2);
-    d.zero(cnt, (LEN_BINS + 2) * sizeof(u32));
+    u32 *hc = d.a->get<u32>(LEN_BINS);
+    d.zero(hc, LEN_BINS * sizeof(u32));
     {
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 4096);
-        k_lms_lens<<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, m, cnt);
+        k_lms_lens<<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, m, hc);
         PSK_LAUNCH_CHECK();
     }
     u32 hist[LEN_BINS];
-    d.read(hist, cnt, sizeof(hist));
-    int D = lms_first_digits(hist, m1, Db);
+    d.read(hist, hc, sizeof(hist));
+    const int D = lms_first_digits(hist, m1, Db);
 
-    u32 *act = d.a->get<u32>(m1), *vt = d.a->get<u32>(m1);
-    u32 *gid = d.a->get<u32>(m1), *gid2 = d.a->get<u32>(m1);
-    u32 *glab = d.a->get<u32>((u64)m1 / 2 + 1), *glab2 = d.a->get<u32>((u64)m1 / 2 + 1);
-    u64 *keys = d.a->get<u64>(m1), *kt = d.a->get<u64>(m1);
-    u32 *L = d.a->get<u32>(m1), *F = d.a->get<u32>(m1);
-    u32 *err = d.err;
-    d.zero(F, (u64)m1 * sizeof(u32));
+    LmsRounds st;
+    st.alloc(d, m1);
+    d.zero(st.F, (u64)m1 * sizeof(u32));
+    u32 *act = st.act;
     for_each(d, m1, [=] __device__(u64 i) { act[i] = (u32)i; });
+    st.A = m1;
+    if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds)) {
+        d.a->reset(mk);
+        return false;
+    }
+    const u32 kk = lms_slot_names(d, st, m1, tails);
+    const u32 *F = st.F, *L = st.L;
+    for_each(d, m1, [=] __device__(u64 r) { reduced[r] = F[L[r]]; });
+    *k = kk;
+    d.a->reset(mk);
+    return true;
+}
 
-    u32 A = m1;
-    u64 off = 0;  // digits compared so far by every active substring
-    int Bg = 0;
-    int rounds = 0;
-    const u64 term = sigma + 1, mm = m;
-    const u32 mm1 = m1;
-    while (A > 0) {
-        if (rounds == max_rounds || (rounds >= LMS_TIED_CHECK_ROUND && A > m1 / 4)) {
+// ---------------------------------------------------------------------------
+// Round 1 by dictionary.  A level-0 text has few distinct LMS substrings (a
+// DNA text of 2^30 bytes has under 10^4 among 3*10^8), so round 1 does not
+// need to sort all m1 keys: the distinct keys are collected in a hash table,
+// sorted, and each substring is counted under its key's dense rank.  A group
+// (equal keys) is finished when its key is terminated or it holds one
+// substring; the substrings of the other groups go on to the sorting rounds
+// with their group's dense id and first slot.
+// ---------------------------------------------------------------------------
+constexpr u64 DICT_EMPTY = ~0ull;  // keys never have bit 63 set
+constexpr int DICT_TBITS = 22;     // insert table of 2^22 slots
+constexpr u32 DICT_MAX_KEYS = 1u << 21;
+constexpr u32 DICT_MAX_PROBES = 128;
+constexpr int DICT_LDS = 4096;     // per-workgroup cache of inserted keys
+constexpr int DICT_BLOCK = 1024;   // ranking workgroups (LDS histogram)
+constexpr u32 DICT_LDS_HIST = 16384;
+constexpr int KEY_WORDS = 9;       // dwords that hold the <= 33 symbols of a key
+
+__device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
+    return (u32)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+// lms_key (o = 0) of a byte text: the key's symbols are read with up to
+// KEY_WORDS aligned dword loads issued together (T must be 4-byte aligned),
+// ranked through an LDS copy of crank.  D <= 4*KEY_WORDS - 3.
+__device__ __forceinline__ u64 lms_key_bytes(const u8 *T, const u8 *lrank, const u32 *lms_text,
+                                             u32 m1, u64 m, u32 r, int D, int Db, u64 term) {
+    u64 p = lms_text[r];
+    bool last = r + 1 == m1;
+    u64 len = last ? m - p : (u64)lms_text[r + 1] - p + 1;
+    u64 td = last ? 0ull : term;
+    const u32 need = (u32)(len < (u64)D ? len : (u64)D);  // symbols in the key
+    const u32 sh = (u32)(p & 3u);
+    const u32 *W = (const u32 *)(T + (p - sh));
+    u32 w[KEY_WORDS];
+    #pragma unroll
+    for (int k = 0; k < KEY_WORDS; k++) w[k] = (u32)(4 * k) < sh + need ? W[k] : 0u;
+    u64 acc = 0;
+    #pragma unroll
+    for (int k = 0; k < KEY_WORDS; k++)
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int j = 4 * k + b - (int)sh;
+            if (j >= 0 && j < (int)need)
+                acc = (acc << Db) | ((u64)lrank[(w[k] >> (8 * b)) & 255u] + 1);
+        }
+    // digits need..D-1: the terminator at index len (if inside), then zeros
+    acc <<= Db * (D - (int)need);
+    if (len < (u64)D) acc |= td << (Db * (D - 1 - (int)len));
+    return (acc << 1) | (len < (u64)D ? 1ull : 0ull);
+}
+
+// Generic keys (unaligned text): one load per symbol.
+__device__ __forceinline__ u64 lms_key_any(const u8 *T, const u8 *lrank, const u32 *lms_text,
+                                           u32 m1, u64 m, u32 r, int D, int Db, u64 term) {
+    struct R {
+        const u8 *T, *lr;
+        __device__ u32 operator()(u64 pos) const { return lr[T[pos]]; }
+    };
+    return lms_key(R{T, lrank}, lms_text, m1, m, r, 0, D, Db, term);
+}
+
+// Inserts every substring's key.  A workgroup remembers the keys it has
+// inserted in LDS (direct mapped), so a frequent key goes to the table once
+// per workgroup.  ctr[0] counts distinct keys, ctr[1] flags a full table.
+template <bool ALIGNED>
+__global__ void __launch_bounds__(BLOCK) k_dict_insert(ByteRanks rk, const u32 *lms_text, u32 m1,
+                                                       u64 m, int D, int Db, u64 term, u64 *tkey,
+                                                       u32 *ctr) {
+    __shared__ u64 seen[DICT_LDS];
+    __shared__ u8 lrank[256];
+    for (int i = threadIdx.x; i < DICT_LDS; i += BLOCK) seen[i] = DICT_EMPTY;
+    lrank[threadIdx.x] = rk.crank[threadIdx.x];
+    __syncthreads();
+    const u32 mask = (1u << DICT_TBITS) - 1u;
+    for (u64 r = (u64)blockIdx.x * BLOCK + threadIdx.x; r < m1; r += (u64)gridDim.x * BLOCK) {
+        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term)
+                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term);
+        u32 s = dict_hash(key, DICT_TBITS);
+        u32 c = s & (DICT_LDS - 1);
+        if (seen[c] == key) continue;
+        bool ok = false;
+        for (u32 pr = 0; pr < DICT_MAX_PROBES; pr++) {
+            u64 cur = __hip_atomic_load(&tkey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) { ok = true; break; }
+            if (cur == DICT_EMPTY) {
+                u64 old = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)DICT_EMPTY,
+                                    (unsigned long long)key);
+                if (old == DICT_EMPTY) {
+                    atomicAdd(&ctr[0], 1u);
+                    ok = true;
+                    break;
+                }
+                if (old == key) { ok = true; break; }
+            }
+            s = (s + 1) & mask;
+        }
+        if (ok) seen[c] = key;
+        else atomicOr(&ctr[1], 1u);
+    }
+}
+
+// Compact table of the K1 distinct keys (2^cbits >= 2*K1 slots) with their
+// dense ranks; every key is inserted once, so slots are claimed by CAS only.
+__global__ void k_dict_compact(const u64 *DK, u32 K1, int cbits, u64 *ckey, u32 *crk, u32 *ctr) {
+    const u32 mask = (1u << cbits) - 1u;
+    for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < K1; j += (u64)gridDim.x * BLOCK) {
+        u64 key = DK[j];
+        u32 s = dict_hash(key, cbits);
+        bool ok = false;
+        for (u32 pr = 0; pr <= mask; pr++) {
+            u64 old = atomicCAS((unsigned long long *)&ckey[s], (unsigned long long)DICT_EMPTY,
+                                (unsigned long long)key);
+            if (old == DICT_EMPTY) {
+                crk[s] = (u32)j;
+                ok = true;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+        if (!ok) atomicOr(&ctr[1], 8u);
+    }
+}
+
+// Dense rank of a key in the compact table (a new kernel sees every insert,
+// so plain loads do); ~0u if absent.
+__device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cbits, u64 key) {
+    const u32 mask = (1u << cbits) - 1u;
+    u32 s = dict_hash(key, cbits);
+    for (u32 pr = 0; pr <= mask; pr++) {
+        u64 cur = ckey[s];
+        if (cur == key) return crk[s];
+        if (cur == DICT_EMPTY) return ~0u;
+        s = (s + 1) & mask;
+    }
+    return ~0u;
+}
+
+// Dense rank of every substring's key into SL[r], and the number of
+// substrings per rank (LDS histogram when the ranks fit, else wave-combined
+// global atomics).  A key missing from the table sets ctr[1].
+template <bool ALIGNED, bool LDSH>
+__global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u32 *lms_text,
+                                                          u32 m1, u64 m, int D, int Db, u64 term,
+                                                          const u64 *ckey, const u32 *crk,
+                                                          int cbits, u32 K1, u32 *SL, u32 *count,
+                                                          u32 *ctr) {
+    __shared__ u32 h[LDSH ? DICT_LDS_HIST : 1];
+    __shared__ u8 lrank[256];
+    if (LDSH)
+        for (u32 i = threadIdx.x; i < K1; i += DICT_BLOCK) h[i] = 0;
+    if (threadIdx.x < 256) lrank[threadIdx.x] = rk.crank[threadIdx.x];
+    __syncthreads();
+    // wave-uniform trip count: every lane reaches the ballots below
+    const u64 stride = (u64)gridDim.x * DICT_BLOCK;
+    for (u64 b = (u64)blockIdx.x * DICT_BLOCK; b < m1; b += stride) {
+        const u64 r = b + threadIdx.x;
+        const bool live = r < m1;
+        u32 dr = 0;
+        if (live) {
+            u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term)
+                              : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term);
+            dr = dict_find(ckey, crk, cbits, key);
+            if (dr >= K1) {
+                atomicOr(&ctr[1], 2u);
+                dr = 0;
+            }
+            SL[r] = dr;
+            if (LDSH) atomicAdd(&h[dr], 1u);
+        }
+        if (!LDSH) {
+            bool pending = live;
+            while (true) {
+                u64 act = ballot(pending);
+                if (!act) break;
+                int leader = __ffsll((long long)act) - 1;
+                u32 dl = __shfl(dr, leader, WAVE);
+                bool mine = pending && dr == dl;
+                u64 peers = ballot(mine);
+                if (lane_id() == leader) atomicAdd(&count[dl], (u32)__popcll(peers));
+                if (mine) pending = false;
+            }
+        }
+    }
+    if (LDSH) {
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < K1; i += DICT_BLOCK)
+            if (h[i]) atomicAdd(&count[i], h[i]);
+    }
+}
+
+// Names of a byte text's LMS substrings through the dictionary; same
+// contract as name_lms_direct.  Returns false, with nothing written, if the
+// keys take more than DICT_MAX_KEYS values or the rounds that follow hit
+// their limits (the caller then uses another strategy).
+inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
+                          u32 *reduced, u32 *tails, u32 *k, int max_rounds) {
+    if (m1 == 0) {
+        *k = 0;
+        return true;
+    }
+    const int Db = bit_width(sigma + 1);
+    const int D = std::min(62 / Db, 4 * KEY_WORDS - 3);
+    if (D < 1) return false;
+    const u64 term = sigma + 1;
+    const bool aligned = (((uintptr_t)rk.T) & 3u) == 0;
+    Arena::Mark mk = d.a->mark();
+    const u64 tsize = 1ull << DICT_TBITS;
+    u64 *tkey = d.a->get<u64>(tsize);
+    u32 *ctr = d.a->get<u32>(4);
+    d.zero(ctr, 4 * sizeof(u32));
+    for_each(d, tsize, [=] __device__(u64 i) { tkey[i] = DICT_EMPTY; });
+    {
+        u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 2048);
+        int pr = d.pbegin("dict_insert", 12.0 * m1);
+        if (aligned)
+            k_dict_insert<true><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
+                                                               tkey, ctr);
+        else
+            k_dict_insert<false><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
+                                                                tkey, ctr);
+        PSK_LAUNCH_CHECK();
+        d.pend(pr);
+    }
+    u32 hc[2];
+    d.read(hc, ctr, sizeof(hc));
+    const u32 K1 = hc[0];
+    if (hc[1] || K1 > DICT_MAX_KEYS || K1 == 0) {
+        d.a->reset(mk);
+        return false;
+    }
+    // distinct keys, sorted, then a compact table of (key, dense rank)
+    u64 *dk = d.a->get<u64>(K1), *dk2 = d.a->get<u64>(K1);
+    u32 *ds = d.a->get<u32>(K1), *ds2 = d.a->get<u32>(K1);
+    scan_exclusive(
+        d, tsize, [=] __device__(u64 i) { return tkey[i] != DICT_EMPTY ? 1u : 0u; },
+        [=] __device__(u64 i, u32 ex, u32 v) {
+            if (v) {
+                dk[ex] = tkey[i];
+                ds[ex] = (u32)i;
+            }
+        },
+        nullptr);
+    bool sw = radix_sort_kv(d, K1, dk, ds, dk2, ds2, 63);
+    const u64 *DK = sw ? dk2 : dk;
+    const int cbits = std::max(10, bit_width((u64)K1 * 2 - 1));
+    u64 *ckey = d.a->get<u64>(1ull << cbits);
+    u32 *crk = d.a->get<u32>(1ull << cbits);
+    for_each(d, 1ull << cbits, [=] __device__(u64 i) { ckey[i] = DICT_EMPTY; });
+    k_dict_compact<<<std::min<u32>(ceil_div(K1, BLOCK), 4096), BLOCK, 0, d.s>>>(DK, K1, cbits, ckey,
+                                                                              crk, ctr);
+    PSK_LAUNCH_CHECK();
+    // rank and count every substring
+    u32 *SL = d.a->get<u32>(m1);
+    u32 *count = d.a->get<u32>(K1 + 1);
+    d.zero(count, ((u64)K1 + 1) * sizeof(u32));
+    {
+        int pr = d.pbegin("dict_rank", 16.0 * m1);
+        const bool lds = K1 <= DICT_LDS_HIST;
+        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), lds ? 512 : 8192);
+        auto go = [&](auto kern) {
+            kern<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, ckey, crk,
+                                                      cbits, K1, SL, count, ctr);
+        };
+        if (aligned && lds) go(k_dict_rank<true, true>);
+        else if (aligned) go(k_dict_rank<true, false>);
+        else if (lds) go(k_dict_rank<false, true>);
+        else go(k_dict_rank<false, false>);
+        PSK_LAUNCH_CHECK();
+        d.pend(pr);
+    }
+    // per rank: first slot (exclusive scan of counts), finished?, and the
+    // dense id among the unfinished groups
+    u32 *start = d.a->get<u32>(K1 + 1);
+    u32 *ugid = d.a->get<u32>(K1 + 1);
+    u32 *ucnt = ctr + 2;  // unfinished groups
+    scan_exclusive(
+        d, K1, [=] __device__(u64 j) { return count[j]; },
+        [=] __device__(u64 j, u32 ex, u32) { start[j] = ex; }, nullptr);
+    auto unfinished = [=] __device__(u64 j) -> bool { return !(DK[j] & 1ull) && count[j] > 1; };
+    scan_exclusive(
+        d, K1, [=] __device__(u64 j) { return unfinished(j) ? 1u : 0u; },
+        [=] __device__(u64 j, u32 ex, u32 v) { ugid[j] = v ? ex : ~0u; }, ucnt);
+    u32 hc2[4];
+    d.read(hc2, ctr, sizeof(hc2));
+    if (hc2[1]) fail(PSK_EDEVICE, "dictionary lookup missed an inserted key");
+    const u32 G = hc2[2];
+
+    LmsRounds st;
+    st.alloc(d, m1);
+    d.zero(st.F, (u64)m1 * sizeof(u32));
+    u32 *F = st.F;
+    // finished groups start at their first slot
+    for_each(d, K1, [=] __device__(u64 j) {
+        if (!unfinished(j)) F[start[j]] = 1;
+    });
+    st.rounds = 1;
+    st.off = (u64)D;
+    if (G) {
+        // the tied substrings, in text order, with their group's dense id;
+        // a group's label is its first slot
+        u32 *act = st.act, *gid = st.gid, *glab = st.glab;
+        for_each(d, K1, [=] __device__(u64 j) {
+            if (ugid[j] != ~0u) glab[ugid[j]] = start[j];
+        });
+        scan_exclusive(
+            d, m1, [=] __device__(u64 r) { return ugid[SL[r]] != ~0u ? 1u : 0u; },
+            [=] __device__(u64 r, u32 ex, u32 v) {
+                if (v) {
+                    act[ex] = (u32)r;
+                    gid[ex] = ugid[SL[r]];
+                }
+            },
+            ctr + 3);
+        st.A = d.read_u32(ctr + 3);
+        st.Bg = bit_width(G - 1);
+        if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds)) {
             d.a->reset(mk);
             return false;
         }
-        rounds++;
-        const bool first = rounds == 1;
-        if (!first) D = (62 - Bg) / Db;
-        const int LB = D * Db + 1;  // digits + finished bit; keys keep <= 63 bits
-        const u64 o = off;
-        const int DD = D, DB = Db;
-        const u32 *ac = act, *gi = gid;
-        for_each(d, A, [=] __device__(u64 i) {
-            u32 r = ac[i];
-            u64 p = lms_text[r];
-            bool last = r + 1 == mm1;
-            u64 len = last ? mm - p : (u64)lms_text[r + 1] - p + 1;
-            u64 td = last ? 0ull : term;
-            u64 acc = 0;
-            for (int j = 0; j < DD; j++) {
-                u64 idx = o + (u64)j;
-                u64 dg = idx < len ? (u64)rk(p + idx) + 1 : (idx == len ? td : 0ull);
-                acc = (acc << DB) | dg;
-            }
-            u64 hi = first ? 0ull : (u64)gi[i];
-            keys[i] = (hi << LB) | (acc << 1) | (len < o + (u64)DD ? 1ull : 0ull);
-        });
-        bool sw = radix_sort_kv(d, A, keys, act, kt, vt, Bg + LB);
-        const u64 *K = sw ? kt : keys;
-        const u32 *V = sw ? vt : act;
-        u32 *nxt = sw ? act : vt;
-        {
-            Arena::Mark m3 = d.a->mark();
-            const u32 nt = ceil_div(A, REL_TILE);
-            u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
-            u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
-            u32 *tq = d.a->get<u32>(nt);
-            RelIn in{K, A, LB, true};
-            k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
-            PSK_LAUNCH_CHECK();
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, cnt);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tq, nt, cnt + 1);
-            PSK_LAUNCH_CHECK();
-            int pr = d.pbegin("lms_name_relabel", 28.0 * A);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, FinishSink{L, F}, nxt, tg, ts, tk, te, tq,
-                                               false, (u64)m1, first ? nullptr : glab, gid2,
-                                               glab2, err);
-            PSK_LAUNCH_CHECK();
-            d.pend(pr);
-            d.a->reset(m3);
-        }
-        u32 ag[2];
-        d.read(ag, cnt, sizeof(ag));
-        A = ag[0];
-        if (nxt != act) std::swap(act, vt);
-        std::swap(gid, gid2);
-        std::swap(glab, glab2);
-        Bg = bit_width(ag[1] ? ag[1] - 1 : 0);
-        off += (u64)D;
     }
-    // slots -> names: the marked group starts, counted in slot order
-    u32 *tl = tails;
-    scan_exclusive(
-        d, m1, [=] __device__(u64 j) { return F[j]; },
-        [=] __device__(u64 j, u32 ex, u32 v) {
-            if (v) {
-                F[j] = ex;
-                if (ex) tl[ex - 1] = (u32)j;
-            }
-        },
-        cnt);
-    const u32 kk = d.read_u32(cnt);
-    for_each(d, 1, [=] __device__(u64) { tl[kk - 1] = mm1; });
-    for_each(d, m1, [=] __device__(u64 r) { reduced[r] = F[L[r]]; });
+    const u32 kk = lms_slot_names(d, st, m1, tails);
+    // names: finished groups from their first slot's mark (a small table),
+    // the refined ones from their own slot
+    u32 *dname = ugid;  // reused: name per rank, ~0u for refined groups
+    for_each(d, K1, [=] __device__(u64 j) { dname[j] = unfinished(j) ? ~0u : F[start[j]]; });
+    const u32 *L = st.L;
+    for_each(d, m1, [=] __device__(u64 r) {
+        u32 v = dname[SL[r]];
+        reduced[r] = v != ~0u ? v : F[L[r]];
+    });
     *k = kk;
     d.a->reset(mk);
     return true;

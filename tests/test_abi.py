@@ -95,6 +95,13 @@ def test_direct_rounds_setting_without_gpu():
             assert e.value.code == P.PSK_EINVAL
 
 
+def test_dict_naming_setting_without_gpu():
+    with P.Context() as c:
+        c.set_dict_naming(False)
+        c.set_dict_naming(True)
+    assert P._lib.psk_ctx_set_dict_naming(None, 1) == P.PSK_EINVAL
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
 def test_no_device_is_an_error_not_an_abort():
     with pytest.raises(P.PskError) as e:

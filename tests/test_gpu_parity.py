@@ -132,11 +132,14 @@ def test_uniform_alphabets(ctx, alphabet):
     assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
 
 
-@pytest.mark.parametrize("rounds", [0, 1, 2, 6])
-def test_naming_strategies_agree(rounds):
+@pytest.mark.parametrize("rounds,dict_naming",
+                         [(0, True), (1, True), (2, True), (6, True), (1, False), (6, False),
+                          (32, False)])
+def test_naming_strategies_agree(rounds, dict_naming):
     # direct LMS-substring sorting with at most `rounds` rounds, induced
-    # sorting where ties remain (0 = induced only), at every level
-    note(f"naming rounds={rounds}")
+    # sorting where ties remain (0 = induced only), at every level; round 1
+    # at level 0 by dictionary or by sorting
+    note(f"naming rounds={rounds} dict={dict_naming}")
     rng = np.random.default_rng(31)
     texts = [gen.markov_dna(1 << 20, seed=9), ADVERSARIAL["descending"], ADVERSARIAL["runs"],
              gen.uniform8(200000, alphabet=3, seed=2), b"abcab" * 20000 + b"x" + b"abcab" * 9000]
@@ -146,6 +149,7 @@ def test_naming_strategies_agree(rounds):
     texts.append(b)
     with P.Context() as c:
         c.set_direct_rounds(rounds)
+        c.set_dict_naming(dict_naming)
         for i, t in enumerate(texts):
             note(f"naming rounds={rounds} text {i}")
             assert np.array_equal(P.psacak(t, c), oracle.sacak8(t)), i
@@ -157,6 +161,28 @@ def test_naming_strategies_agree(rounds):
                 assert np.array_equal(P.psacak32(tr, c), oracle.sacak32(tr)), it
         z = gen.zipf32(1 << 18, sigma=1 << 12, s=1.1, repeat_frac=0.3, seed=3)
         assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
+
+
+@pytest.mark.parametrize("case", ["long_runs", "many_keys", "skewed"])
+def test_dictionary_naming_cases(case):
+    # level-0 naming through the dictionary: long LMS substrings left tied
+    # after round 1 (homopolymer runs), more distinct keys than the
+    # dictionary holds (falls back to sorting), and a few very frequent keys
+    rng = np.random.default_rng(5)
+    if case == "long_runs":
+        parts = [bytes([int(rng.choice(list(b"ACGT")))]) * int(rng.integers(1, 90))
+                 for _ in range(40000)]
+        t = np.frombuffer(b"".join(parts), dtype=np.uint8)
+    elif case == "many_keys":
+        t = gen.uniform8(1 << 23, alphabet=256, seed=77)
+    else:
+        t = np.frombuffer((b"CA" * 300000) + gen.iid_dna(1 << 20, seed=4).tobytes(), dtype=np.uint8)
+    note(f"dictionary case={case} n={len(t)}")
+    with P.Context() as c:
+        got = P.psacak(t, c)
+        c.set_dict_naming(False)
+        assert np.array_equal(P.psacak(t, c), got)
+    assert np.array_equal(got, oracle.sacak8(t))
 
 
 def test_psacak32_random_bucket_pointer_texts(ctx):
