@@ -38,13 +38,16 @@ def _run(cmd):
     return out
 
 
-def build_hip(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    out = _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "psacak.hip")])
-    os.replace(tmp, LIB)
+def build_hip(verbose: bool = False, out_path: str = LIB, defines=()) -> str:
+    """Compile the library; defines (e.g. "PSK_RS_ROUNDS=16") override tile
+    constants for tuning builds."""
+    tmp = out_path + ".tmp"
+    out = _run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-o", tmp,
+                os.path.join(CSRC, "psacak.hip")])
+    os.replace(tmp, out_path)
     if verbose and out.strip():
         print(out)
-    return LIB
+    return out_path
 
 
 def build_gen() -> str:

@@ -260,7 +260,10 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
     return x * q + (x < r ? x : r) + i;
 }
 
-constexpr int MS_ROUNDS = 16;
+#ifndef PSK_MS_ROUNDS
+#define PSK_MS_ROUNDS 16
+#endif
+constexpr int MS_ROUNDS = PSK_MS_ROUNDS;
 constexpr u64 MS_TILE = (u64)BLOCK * MS_ROUNDS;
 
 template <class Dig>
@@ -374,12 +377,18 @@ inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char
 // contiguous, coalesced run.
 // Returns true if the sorted result ended in (kt, vt) rather than (k, v).
 // ---------------------------------------------------------------------------
-constexpr int RS_ROUNDS = 8;
+#ifndef PSK_RS_ROUNDS
+#define PSK_RS_ROUNDS 8
+#endif
+#ifndef PSK_RS_CT
+#define PSK_RS_CT 1
+#endif
+constexpr int RS_ROUNDS = PSK_RS_ROUNDS;
 constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
 
 // Counting: one block per RS_CT consecutive tiles (fewer, longer-lived
 // workgroups; each digit's RS_CT counts are written as one contiguous run).
-constexpr u32 RS_CT = 1;
+constexpr u32 RS_CT = PSK_RS_CT;
 __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, u64 n, int shift,
                                                     u32 *hist, u32 nt) {
     __shared__ u32 h[RS_CT][256];
