@@ -266,6 +266,25 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
 constexpr int MS_ROUNDS = PSK_MS_ROUNDS;
 constexpr u64 MS_TILE = (u64)BLOCK * MS_ROUNDS;
 
+// LDS count of one digit per lane: up to 4 distinct digits of the wave are
+// added by one lane each (few distinct digits are the common case for byte
+// buckets), the rest with plain atomics.
+__device__ __forceinline__ void lds_count_wave(u32 *h, u32 x, bool valid) {
+    bool pending = valid;
+    #pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const u64 act = ballot(pending);
+        if (!act) return;
+        const int leader = __ffsll((long long)act) - 1;
+        const u32 xl = __shfl(x, leader, WAVE);
+        const bool mine = pending && x == xl;
+        const u64 peers = ballot(mine);
+        if (lane_id() == leader) atomicAdd(&h[xl], (u32)__popcll(peers));
+        if (mine) pending = false;
+    }
+    if (pending) atomicAdd(&h[x], 1u);
+}
+
 template <class Dig>
 __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
     __shared__ u32 h[256];
@@ -277,7 +296,7 @@ __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         u32 key = 0;
         bool v = (i < n) && dig(i, &key);
-        if (v) atomicAdd(&h[key], 1u);
+        lds_count_wave(h, key, v);
     }
     __syncthreads();
     hist[(u64)threadIdx.x * nt + t] = h[threadIdx.x];
@@ -406,12 +425,21 @@ __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, 
             u64 i = (u64)(t0 + c) * RS_TILE + (u64)r * BLOCK + threadIdx.x;
             dk[c][r] = i < n ? (u32)(ki[i] >> shift) & 255u : 256u;
         }
-    // plain LDS atomics: cheaper here than aggregating equal digits per wave
+    // plain LDS atomics, except when the whole wave holds one digit (the high
+    // digits of grouped keys): then one lane adds the wave's count, instead
+    // of 64 lanes serialising on one counter
     #pragma unroll
     for (u32 c = 0; c < RS_CT; c++)
         #pragma unroll
-        for (int r = 0; r < RS_ROUNDS; r++)
-            if (dk[c][r] < 256u) atomicAdd(&h[c][dk[c][r]], 1u);
+        for (int r = 0; r < RS_ROUNDS; r++) {
+            const u32 x = dk[c][r];
+            const u32 x0 = __shfl(x, 0, WAVE);
+            if (ballot(x == x0) == ~0ull) {
+                if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[c][x0], (u32)WAVE);
+            } else if (x < 256u) {
+                atomicAdd(&h[c][x], 1u);
+            }
+        }
     __syncthreads();
     for (u32 c = 0; c < RS_CT && t0 + c < nt; c++)
         hist[(u64)threadIdx.x * nt + t0 + c] = h[c][threadIdx.x];
