@@ -221,11 +221,14 @@ def bench_batch(args):
         P.batch8_dev(d_text, off[:101], d_sa, ctx=ctx)
     torch.cuda.synchronize()
     log(f"timing batched x{args.steps}")
+    P.kernel_profile(ctx, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         P.batch8_dev(d_text, off, d_sa, ctx=ctx)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    kstats = P.kernel_stats(ctx)
+    P.kernel_profile(ctx, False)
     ms = (t1 - t0) * 1000.0 / args.steps
     # per-text plain calls on a bounded sample of the same texts
     sample = min(count, args.per_text_sample)
@@ -275,6 +278,9 @@ def bench_batch(args):
                      "speedup_full_vs_per_text": round(gbps / pertext, 2)},
         "parity": {"device_check_all_texts": ok_all, "batched_equals_plain_on_sample": same_as_plain,
                    "oracle_equal_small_texts": bool(oracle_ok), "oracle_checked": len(small)},
+        "kernels": sorted(
+            [{"name": k["name"], "launches": k["launches"], "ms": round(k["ms"] / args.steps, 3)}
+             for k in kstats], key=lambda k: -k["ms"]),
     }
     print(json.dumps(out), flush=True)
     ctx.close()

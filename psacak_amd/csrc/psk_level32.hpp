@@ -300,7 +300,8 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
 // Returns false if the substrings need more rounds (nothing is written).
 template <class RF>
 inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32 *lms_text,
-                                 u32 m1, u32 *SA1, int depth, LevelStat &st) {
+                                 u32 m1, u32 *SA1, int depth, LevelStat &st,
+                                 LmsEnds ends = LmsEnds{}) {
     Dev &d = rt.d;
     if (m1 == 0) return true;
     if (rt.direct_rounds == 0) return false;
@@ -312,10 +313,10 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     // dictionary first there
     bool dict = false;
     if constexpr (std::is_same<RF, ByteRanks>::value)
-        dict = rt.dict_naming &&
-               name_lms_dict(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds);
-    if (!dict &&
-        !name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k, rt.direct_rounds)) {
+        dict = rt.dict_naming && name_lms_dict(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
+                                               rt.direct_rounds, ends);
+    if (!dict && !name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
+                                  rt.direct_rounds, ends)) {
         d.a->reset(mk);
         return false;
     }

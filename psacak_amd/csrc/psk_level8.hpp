@@ -120,8 +120,18 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // ---- LMS-substring names and the reduced problem (psacak8.rs:33-48) ------
     u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
     bool named = false;
-    if (!EB) {
-        // by sorting the substrings, bytes ranked among those present
+    {
+        // by sorting the substrings, bytes ranked among those present; in a
+        // batch each text's last substring runs into its own sentinel
+        LmsEnds ends;
+        if (EB) {
+            u32 *EBR = d.a->get<u32>(nwords + 1);
+            scan_exclusive(
+                d, nwords, [=] __device__(u64 w) { return (u32)__popc(EB[w]); },
+                [=] __device__(u64 w, u32 ex, u32) { EBR[w] = ex; }, nullptr);
+            ends.EB = EB;
+            ends.EBR = EBR;
+        }
         u8 hcr[256];
         u32 sigma = 0;
         for (int c = 0; c < 256; c++) {
@@ -130,13 +140,14 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         }
         u8 *crank = d.a->get<u8>(256);
         d.write(crank, hcr, sizeof(hcr));
-        named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st);
+        named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st,
+                                     ends);
     }
     if (!named) {
-        // induced stage 1 (batches, whose substrings that cross a text end must
-        // stay unique, and long repeated LMS substrings): LMS characters to
-        // bucket tails (psacak8.rs:57-72), in text order within a bucket, then
-        // induce the LMS substrings
+        // induced stage 1 (long repeated LMS substrings, or batch texts whose
+        // index does not fit a key): LMS characters to bucket tails
+        // (psacak8.rs:57-72), in text order within a bucket, then induce the
+        // LMS substrings
         multisplit(
             d, m1,
             [=] __device__(u64 i, u32 *key) {

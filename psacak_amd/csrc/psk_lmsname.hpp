@@ -100,23 +100,82 @@ inline int lms_first_digits(const u32 *hist, u32 m1, int Db) {
 // repeated substrings (induced sorting is then the cheaper way).
 constexpr int LMS_TIED_CHECK_ROUND = 4;
 
+// Texts of a batch (concatenated, each with its own sentinel): EB marks every
+// text's last position, EBR[w] counts the marks in EB words before w.  The
+// LMS substring that reaches its text's last position runs into that text's
+// sentinel, a symbol below every other; the sentinels of the batch rank in
+// text order.  Null EB: one text.
+struct LmsEnds {
+    const u32 *EB = nullptr;
+    const u32 *EBR = nullptr;
+    u32 *flag = nullptr;  // set when a key cannot hold its text's index
+};
+
+// Length of the r-th LMS substring (symbols, terminal included).  isend: it
+// runs into a sentinel instead of ending at the next LMS position (the last
+// substring of a text); tid: then the index of its text.
+__device__ __forceinline__ u64 lms_span(const u32 *lms_text, u32 m1, u64 m, u32 r,
+                                        const LmsEnds &E, bool &isend, u32 &tid) {
+    const u64 p = lms_text[r];
+    const bool last = r + 1 == m1;
+    const u64 q = last ? m : (u64)lms_text[r + 1];
+    tid = 0;
+    if (E.EB) {
+        // first text end at or after p, before q (a text end is L-type, so it
+        // is never q itself); the last text ends at m - 1
+        u64 w = p >> 5;
+        const u64 wl = (q - 1) >> 5;
+        u32 x = E.EB[w] & (0xFFFFFFFFu << (p & 31));
+        while (!x && w < wl) x = E.EB[++w];
+        if (x) {
+            const u64 e = w * 32 + (u64)(__ffs(x) - 1);
+            if (e < q) {
+                isend = true;
+                tid = E.EBR[w] + (u32)__popc(E.EB[w] & ((1u << (e & 31)) - 1u));
+                return e - p + 1;
+            }
+        }
+        isend = false;
+        return q - p + 1;
+    }
+    isend = last;
+    return last ? m - p : q - p + 1;
+}
+
+// A substring that runs into a sentinel is unique: among substrings with the
+// same symbols the sentinels' order (text order) decides.  The digits after
+// the sentinel digit are zeros in every key, so the text index goes there,
+// in the round whose digits [o, o+D) hold the sentinel.
+__device__ __forceinline__ u64 lms_end_index(u64 acc, u64 len, u64 o, int D, int Db, u32 tid,
+                                             const LmsEnds &E) {
+    if (tid == 0 || len < o || len >= o + (u64)D) return acc;
+    const int bits = Db * (int)(o + (u64)D - 1 - len);
+    if (bits < 32 && (tid >> bits) != 0) {
+        if (E.flag) atomicOr(E.flag, 1u);
+        return acc;
+    }
+    return acc | (u64)tid;
+}
+
 // Key of the r-th LMS substring for a round that compares digits [o, o+D):
 // the digits packed Db bits each (first digit highest), then the finished
-// bit (the terminator, or the sentinel digit of the last substring, lies
+// bit (the terminator, or the sentinel digit of a text's last substring, lies
 // within these digits).  D*Db + 1 <= 63, so a key never has bit 63 set.
 template <class RF>
 __device__ __forceinline__ u64 lms_key(RF rk, const u32 *lms_text, u32 m1, u64 m, u32 r, u64 o,
-                                       int D, int Db, u64 term) {
-    u64 p = lms_text[r];
-    bool last = r + 1 == m1;
-    u64 len = last ? m - p : (u64)lms_text[r + 1] - p + 1;
-    u64 td = last ? 0ull : term;
+                                       int D, int Db, u64 term, const LmsEnds &E) {
+    bool isend;
+    u32 tid;
+    const u64 len = lms_span(lms_text, m1, m, r, E, isend, tid);
+    const u64 p = lms_text[r];
+    const u64 td = isend ? 0ull : term;
     u64 acc = 0;
     for (int j = 0; j < D; j++) {
         u64 idx = o + (u64)j;
         u64 dg = idx < len ? (u64)rk(p + idx) + 1 : (idx == len ? td : 0ull);
         acc = (acc << Db) | dg;
     }
+    if (isend) acc = lms_end_index(acc, len, o, D, Db, tid, E);
     return (acc << 1) | (len < o + (u64)D ? 1ull : 0ull);
 }
 
@@ -144,7 +203,7 @@ struct LmsRounds {
         kt = d.a->get<u64>(m1);
         L = d.a->get<u32>(m1);
         F = d.a->get<u32>(m1);
-        cnt = d.a->get<u32>(2);
+        cnt = d.a->get<u32>(4);
     }
 };
 
@@ -153,7 +212,7 @@ struct LmsRounds {
 // Returns false when the round limits are reached with ties left.
 template <class RF>
 inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32 *lms_text,
-                       u32 m1, int D1, int max_rounds) {
+                       u32 m1, int D1, int max_rounds, LmsEnds E) {
     const int Db = bit_width(sigma + 1);
     const u64 term = sigma + 1, mm = m;
     const u32 mm1 = m1;
@@ -172,7 +231,7 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
         for_each(d, st.A, [=] __device__(u64 i) {
             u32 r = ac[i];
             u64 hi = first ? 0ull : (u64)gi[i];
-            keys[i] = (hi << LB) | lms_key(rk, lms_text, mm1, mm, r, o, DD, DB, term);
+            keys[i] = (hi << LB) | lms_key(rk, lms_text, mm1, mm, r, o, DD, DB, term, E);
         });
         bool sw = radix_sort_kv(d, st.A, st.keys, st.act, st.kt, st.vt, st.Bg + LB);
         const u64 *K = sw ? st.kt : st.keys;
@@ -238,7 +297,8 @@ inline u32 lms_slot_names(Dev &d, LmsRounds &st, u32 m1, u32 *tails) {
 // the substrings are still tied after LMS_TIED_CHECK_ROUND rounds.
 template <class RF>
 inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
-                            u32 *reduced, u32 *tails, u32 *k, int max_rounds) {
+                            u32 *reduced, u32 *tails, u32 *k, int max_rounds,
+                            LmsEnds E = LmsEnds{}) {
     if (m1 == 0) {
         *k = 0;
         return true;
@@ -262,7 +322,12 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
     u32 *act = st.act;
     for_each(d, m1, [=] __device__(u64 i) { act[i] = (u32)i; });
     st.A = m1;
-    if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds)) {
+    if (E.EB) {
+        E.flag = st.cnt + 2;  // cnt[0..1] belong to the rounds
+        d.zero(E.flag, sizeof(u32));
+    }
+    if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||
+        (E.EB && d.read_u32(E.flag))) {
         d.a->reset(mk);
         return false;
     }
@@ -289,7 +354,8 @@ constexpr u32 DICT_MAX_KEYS = 1u << 21;
 constexpr u32 DICT_MAX_PROBES = 128;
 constexpr int DICT_LDS = 4096;     // per-workgroup cache of inserted keys
 constexpr int DICT_BLOCK = 1024;   // ranking workgroups (LDS histogram)
-constexpr u32 DICT_LDS_HIST = 16384;
+constexpr u32 DICT_HIST_SMALL = 16384;  // 64 KB of LDS
+constexpr u32 DICT_HIST_BIG = 32768;    // 128 KB (devices with 160 KB per workgroup)
 constexpr int KEY_WORDS = 9;       // dwords that hold the <= 33 symbols of a key
 
 __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
@@ -300,11 +366,13 @@ __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
 // KEY_WORDS aligned dword loads issued together (T must be 4-byte aligned),
 // ranked through an LDS copy of crank.  D <= 4*KEY_WORDS - 3.
 __device__ __forceinline__ u64 lms_key_bytes(const u8 *T, const u8 *lrank, const u32 *lms_text,
-                                             u32 m1, u64 m, u32 r, int D, int Db, u64 term) {
-    u64 p = lms_text[r];
-    bool last = r + 1 == m1;
-    u64 len = last ? m - p : (u64)lms_text[r + 1] - p + 1;
-    u64 td = last ? 0ull : term;
+                                             u32 m1, u64 m, u32 r, int D, int Db, u64 term,
+                                             const LmsEnds &E) {
+    bool isend;
+    u32 tid;
+    const u64 len = lms_span(lms_text, m1, m, r, E, isend, tid);
+    const u64 p = lms_text[r];
+    const u64 td = isend ? 0ull : term;
     const u32 need = (u32)(len < (u64)D ? len : (u64)D);  // symbols in the key
     const u32 sh = (u32)(p & 3u);
     const u32 *W = (const u32 *)(T + (p - sh));
@@ -323,17 +391,19 @@ __device__ __forceinline__ u64 lms_key_bytes(const u8 *T, const u8 *lrank, const
     // digits need..D-1: the terminator at index len (if inside), then zeros
     acc <<= Db * (D - (int)need);
     if (len < (u64)D) acc |= td << (Db * (D - 1 - (int)len));
+    if (isend) acc = lms_end_index(acc, len, 0, D, Db, tid, E);
     return (acc << 1) | (len < (u64)D ? 1ull : 0ull);
 }
 
 // Generic keys (unaligned text): one load per symbol.
 __device__ __forceinline__ u64 lms_key_any(const u8 *T, const u8 *lrank, const u32 *lms_text,
-                                           u32 m1, u64 m, u32 r, int D, int Db, u64 term) {
+                                           u32 m1, u64 m, u32 r, int D, int Db, u64 term,
+                                           const LmsEnds &E) {
     struct R {
         const u8 *T, *lr;
         __device__ u32 operator()(u64 pos) const { return lr[T[pos]]; }
     };
-    return lms_key(R{T, lrank}, lms_text, m1, m, r, 0, D, Db, term);
+    return lms_key(R{T, lrank}, lms_text, m1, m, r, 0, D, Db, term, E);
 }
 
 // Inserts every substring's key.  A workgroup remembers the keys it has
@@ -341,8 +411,8 @@ __device__ __forceinline__ u64 lms_key_any(const u8 *T, const u8 *lrank, const u
 // per workgroup.  ctr[0] counts distinct keys, ctr[1] flags a full table.
 template <bool ALIGNED>
 __global__ void __launch_bounds__(BLOCK) k_dict_insert(ByteRanks rk, const u32 *lms_text, u32 m1,
-                                                       u64 m, int D, int Db, u64 term, u64 *tkey,
-                                                       u32 *ctr) {
+                                                       u64 m, int D, int Db, u64 term, LmsEnds E,
+                                                       u64 *tkey, u32 *ctr) {
     __shared__ u64 seen[DICT_LDS];
     __shared__ u8 lrank[256];
     for (int i = threadIdx.x; i < DICT_LDS; i += BLOCK) seen[i] = DICT_EMPTY;
@@ -350,8 +420,8 @@ __global__ void __launch_bounds__(BLOCK) k_dict_insert(ByteRanks rk, const u32 *
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
     for (u64 r = (u64)blockIdx.x * BLOCK + threadIdx.x; r < m1; r += (u64)gridDim.x * BLOCK) {
-        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term)
-                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term);
+        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
+                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
         u32 s = dict_hash(key, DICT_TBITS);
         u32 c = s & (DICT_LDS - 1);
         if (seen[c] == key) continue;
@@ -413,18 +483,18 @@ __device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cb
 }
 
 // Dense rank of every substring's key into SL[r], and the number of
-// substrings per rank (LDS histogram when the ranks fit, else wave-combined
-// global atomics).  A key missing from the table sets ctr[1].
-template <bool ALIGNED, bool LDSH>
+// substrings per rank: ranks below HCAP in an LDS histogram, the others with
+// wave-combined global atomics.  A key missing from the table sets ctr[1].
+template <bool ALIGNED, u32 HCAP>
 __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u32 *lms_text,
                                                           u32 m1, u64 m, int D, int Db, u64 term,
-                                                          const u64 *ckey, const u32 *crk,
+                                                          LmsEnds E, const u64 *ckey, const u32 *crk,
                                                           int cbits, u32 K1, u32 *SL, u32 *count,
                                                           u32 *ctr) {
-    __shared__ u32 h[LDSH ? DICT_LDS_HIST : 1];
+    __shared__ u32 h[HCAP];
     __shared__ u8 lrank[256];
-    if (LDSH)
-        for (u32 i = threadIdx.x; i < K1; i += DICT_BLOCK) h[i] = 0;
+    const u32 hn = K1 < HCAP ? K1 : HCAP;
+    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) h[i] = 0;
     if (threadIdx.x < 256) lrank[threadIdx.x] = rk.crank[threadIdx.x];
     __syncthreads();
     // wave-uniform trip count: every lane reaches the ballots below
@@ -434,35 +504,31 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u3
         const bool live = r < m1;
         u32 dr = 0;
         if (live) {
-            u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term)
-                              : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term);
+            u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
+                              : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
             dr = dict_find(ckey, crk, cbits, key);
             if (dr >= K1) {
                 atomicOr(&ctr[1], 2u);
                 dr = 0;
             }
             SL[r] = dr;
-            if (LDSH) atomicAdd(&h[dr], 1u);
+            if (dr < HCAP) atomicAdd(&h[dr], 1u);
         }
-        if (!LDSH) {
-            bool pending = live;
-            while (true) {
-                u64 act = ballot(pending);
-                if (!act) break;
-                int leader = __ffsll((long long)act) - 1;
-                u32 dl = __shfl(dr, leader, WAVE);
-                bool mine = pending && dr == dl;
-                u64 peers = ballot(mine);
-                if (lane_id() == leader) atomicAdd(&count[dl], (u32)__popcll(peers));
-                if (mine) pending = false;
-            }
+        bool pending = live && dr >= HCAP;
+        while (true) {
+            u64 act = ballot(pending);
+            if (!act) break;
+            int leader = __ffsll((long long)act) - 1;
+            u32 dl = __shfl(dr, leader, WAVE);
+            bool mine = pending && dr == dl;
+            u64 peers = ballot(mine);
+            if (lane_id() == leader) atomicAdd(&count[dl], (u32)__popcll(peers));
+            if (mine) pending = false;
         }
     }
-    if (LDSH) {
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < K1; i += DICT_BLOCK)
-            if (h[i]) atomicAdd(&count[i], h[i]);
-    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK)
+        if (h[i]) atomicAdd(&count[i], h[i]);
 }
 
 // Names of a byte text's LMS substrings through the dictionary; same
@@ -470,7 +536,8 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u3
 // keys take more than DICT_MAX_KEYS values or the rounds that follow hit
 // their limits (the caller then uses another strategy).
 inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
-                          u32 *reduced, u32 *tails, u32 *k, int max_rounds) {
+                          u32 *reduced, u32 *tails, u32 *k, int max_rounds,
+                          LmsEnds E = LmsEnds{}) {
     if (m1 == 0) {
         *k = 0;
         return true;
@@ -485,16 +552,17 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
     u64 *tkey = d.a->get<u64>(tsize);
     u32 *ctr = d.a->get<u32>(4);
     d.zero(ctr, 4 * sizeof(u32));
+    E.flag = ctr + 1;  // a text index that does not fit counts as a failure
     for_each(d, tsize, [=] __device__(u64 i) { tkey[i] = DICT_EMPTY; });
     {
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 2048);
         int pr = d.pbegin("dict_insert", 12.0 * m1);
         if (aligned)
             k_dict_insert<true><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
-                                                               tkey, ctr);
+                                                               E, tkey, ctr);
         else
             k_dict_insert<false><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
-                                                                tkey, ctr);
+                                                                E, tkey, ctr);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
     }
@@ -532,16 +600,18 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
     d.zero(count, ((u64)K1 + 1) * sizeof(u32));
     {
         int pr = d.pbegin("dict_rank", 16.0 * m1);
-        const bool lds = K1 <= DICT_LDS_HIST;
-        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), lds ? 512 : 8192);
+        // one workgroup per CU with the large histogram where the device
+        // gives a workgroup that much LDS, two with the small one
+        const bool big = d.lds_per_block >= DICT_HIST_BIG * 4 + 1024 && K1 > DICT_HIST_SMALL;
+        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), big ? 256 : 512);
         auto go = [&](auto kern) {
-            kern<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, ckey, crk,
-                                                      cbits, K1, SL, count, ctr);
+            kern<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E, ckey,
+                                                      crk, cbits, K1, SL, count, ctr);
         };
-        if (aligned && lds) go(k_dict_rank<true, true>);
-        else if (aligned) go(k_dict_rank<true, false>);
-        else if (lds) go(k_dict_rank<false, true>);
-        else go(k_dict_rank<false, false>);
+        if (aligned && big) go(k_dict_rank<true, DICT_HIST_BIG>);
+        else if (aligned) go(k_dict_rank<true, DICT_HIST_SMALL>);
+        else if (big) go(k_dict_rank<false, DICT_HIST_BIG>);
+        else go(k_dict_rank<false, DICT_HIST_SMALL>);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
     }
@@ -590,7 +660,10 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
             ctr + 3);
         st.A = d.read_u32(ctr + 3);
         st.Bg = bit_width(G - 1);
-        if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds)) {
+        E.flag = ctr + 3;  // free again after the count of tied substrings
+        d.zero(E.flag, sizeof(u32));
+        if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||
+            (E.EB && d.read_u32(E.flag))) {
             d.a->reset(mk);
             return false;
         }

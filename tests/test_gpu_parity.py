@@ -163,11 +163,12 @@ def test_naming_strategies_agree(rounds, dict_naming):
         assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
 
 
-@pytest.mark.parametrize("case", ["long_runs", "many_keys", "skewed"])
+@pytest.mark.parametrize("case", ["long_runs", "many_keys", "mid_keys", "skewed"])
 def test_dictionary_naming_cases(case):
     # level-0 naming through the dictionary: long LMS substrings left tied
     # after round 1 (homopolymer runs), more distinct keys than the
-    # dictionary holds (falls back to sorting), and a few very frequent keys
+    # dictionary holds (falls back to sorting), more than its LDS histogram
+    # holds, and a few very frequent keys
     rng = np.random.default_rng(5)
     if case == "long_runs":
         parts = [bytes([int(rng.choice(list(b"ACGT")))]) * int(rng.integers(1, 90))
@@ -175,6 +176,9 @@ def test_dictionary_naming_cases(case):
         t = np.frombuffer(b"".join(parts), dtype=np.uint8)
     elif case == "many_keys":
         t = gen.uniform8(1 << 23, alphabet=256, seed=77)
+    elif case == "mid_keys":
+        # ~7*10^4 distinct keys: LDS histogram plus global counts
+        t = gen.uniform8(1 << 22, alphabet=8, seed=77)
     else:
         t = np.frombuffer((b"CA" * 300000) + gen.iid_dna(1 << 20, seed=4).tobytes(), dtype=np.uint8)
     note(f"dictionary case={case} n={len(t)}")
@@ -262,6 +266,36 @@ def test_batch_many_small_and_dev(ctx):
     sa = ds.cpu().numpy().view(np.uint32)
     for i, t in enumerate(texts):
         assert np.array_equal(sa[off[i]:off[i + 1]], oracle.sacak8(t)), i
+
+
+@pytest.mark.parametrize("dict_naming,rounds", [(True, 32), (False, 32), (True, 1), (False, 0)])
+def test_batch_text_ends(dict_naming, rounds):
+    # each text's last LMS substring runs into its own sentinel: identical
+    # tails in many texts (equal symbols, distinct sentinels), tails longer
+    # than one round's digits, thousands of texts (wide text indices), tiny
+    # and empty texts; every naming strategy gives each text its own SA
+    note(f"batch text ends dict={dict_naming} rounds={rounds}")
+    rng = np.random.default_rng(41)
+    base = [b"GATTACA", b"CAGCAG", b"ACGTACGA", b"TTTTTTTTTTTTTTTTTTTTTTTTTTTTTA", b"CA", b"A", b""]
+    texts = []
+    for i in range(4000):
+        k = int(rng.integers(0, 6))
+        if k == 0:
+            texts.append(base[int(rng.integers(0, len(base)))])
+        elif k == 1:
+            texts.append(gen.iid_dna(int(rng.integers(1, 40)), seed=i).tobytes()
+                         + base[int(rng.integers(0, 4))])
+        elif k == 2:
+            texts.append(gen.markov_dna(int(rng.integers(50, 3000)), seed=i).tobytes()
+                         + b"A" * int(rng.integers(1, 40)))
+        else:
+            texts.append(gen.iid_dna(int(rng.integers(1, 600)), seed=7 * i).tobytes())
+    with P.Context() as c:
+        c.set_dict_naming(dict_naming)
+        c.set_direct_rounds(rounds)
+        outs = P.batch8(texts, c)
+    for i, (t, o) in enumerate(zip(texts, outs)):
+        assert np.array_equal(o, oracle.sacak8(np.frombuffer(t, dtype=np.uint8))), i
 
 
 def test_batch_matches_single_calls_on_repeats(ctx):
