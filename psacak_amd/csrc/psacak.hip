@@ -47,11 +47,6 @@ void ensure_ready(psk_ctx *c) {
     c->rt.d.a = &c->arena;
     c->rt.d.pinned = c->pinned;
     c->rt.d.prof = &c->prof;
-    int lds = 0;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) ==
-            hipSuccess &&
-        lds > 0)
-        c->rt.d.lds_per_block = (u32)lds;
     c->ready = true;
 }
 

@@ -354,8 +354,8 @@ constexpr u32 DICT_MAX_KEYS = 1u << 21;
 constexpr u32 DICT_MAX_PROBES = 128;
 constexpr int DICT_LDS = 4096;     // per-workgroup cache of inserted keys
 constexpr int DICT_BLOCK = 1024;   // ranking workgroups (LDS histogram)
-constexpr u32 DICT_HIST_SMALL = 16384;  // 64 KB of LDS
-constexpr u32 DICT_HIST_BIG = 32768;    // 128 KB (devices with 160 KB per workgroup)
+constexpr u32 DICT_HIST = 16384;   // ranks per LDS histogram window (64 KB)
+constexpr u32 DICT_WINDOWS = 8;    // windows before the rest is counted globally
 constexpr int KEY_WORDS = 9;       // dwords that hold the <= 33 symbols of a key
 
 __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
@@ -483,52 +483,71 @@ __device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cb
 }
 
 // Dense rank of every substring's key into SL[r], and the number of
-// substrings per rank: ranks below HCAP in an LDS histogram, the others with
-// wave-combined global atomics.  A key missing from the table sets ctr[1].
-template <bool ALIGNED, u32 HCAP>
+// substrings per rank below DICT_HIST (an LDS histogram per workgroup).  A
+// key missing from the table sets ctr[1].
+template <bool ALIGNED>
 __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u32 *lms_text,
                                                           u32 m1, u64 m, int D, int Db, u64 term,
                                                           LmsEnds E, const u64 *ckey, const u32 *crk,
                                                           int cbits, u32 K1, u32 *SL, u32 *count,
                                                           u32 *ctr) {
-    __shared__ u32 h[HCAP];
+    __shared__ u32 h[DICT_HIST];
     __shared__ u8 lrank[256];
-    const u32 hn = K1 < HCAP ? K1 : HCAP;
+    const u32 hn = K1 < DICT_HIST ? K1 : DICT_HIST;
     for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) h[i] = 0;
     if (threadIdx.x < 256) lrank[threadIdx.x] = rk.crank[threadIdx.x];
+    __syncthreads();
+    for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
+         r += (u64)gridDim.x * DICT_BLOCK) {
+        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
+                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
+        u32 dr = dict_find(ckey, crk, cbits, key);
+        if (dr >= K1) {
+            atomicOr(&ctr[1], 2u);
+            dr = 0;
+        }
+        SL[r] = dr;
+        if (dr < DICT_HIST) atomicAdd(&h[dr], 1u);
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK)
+        if (h[i]) atomicAdd(&count[i], h[i]);
+}
+
+// Counts of the ranks in [lo, lo + DICT_HIST) from SL (LDS histogram); with
+// TAIL also every rank above that window (wave-combined global atomics, for
+// dictionaries too large for a few windows).
+template <bool TAIL>
+__global__ void __launch_bounds__(DICT_BLOCK) k_dict_count(const u32 *SL, u32 m1, u32 lo, u32 K1,
+                                                           u32 *count) {
+    __shared__ u32 h[DICT_HIST];
+    const u32 hn = K1 - lo < DICT_HIST ? K1 - lo : DICT_HIST;
+    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) h[i] = 0;
     __syncthreads();
     // wave-uniform trip count: every lane reaches the ballots below
     const u64 stride = (u64)gridDim.x * DICT_BLOCK;
     for (u64 b = (u64)blockIdx.x * DICT_BLOCK; b < m1; b += stride) {
         const u64 r = b + threadIdx.x;
-        const bool live = r < m1;
-        u32 dr = 0;
-        if (live) {
-            u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
-                              : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
-            dr = dict_find(ckey, crk, cbits, key);
-            if (dr >= K1) {
-                atomicOr(&ctr[1], 2u);
-                dr = 0;
+        const u32 dr = r < m1 ? SL[r] : 0u;
+        const bool in = r < m1 && dr >= lo && dr - lo < DICT_HIST;
+        if (in) atomicAdd(&h[dr - lo], 1u);
+        if (TAIL) {
+            bool pending = r < m1 && dr >= lo && dr - lo >= DICT_HIST;
+            while (true) {
+                u64 act = ballot(pending);
+                if (!act) break;
+                int leader = __ffsll((long long)act) - 1;
+                u32 dl = __shfl(dr, leader, WAVE);
+                bool mine = pending && dr == dl;
+                u64 peers = ballot(mine);
+                if (lane_id() == leader) atomicAdd(&count[dl], (u32)__popcll(peers));
+                if (mine) pending = false;
             }
-            SL[r] = dr;
-            if (dr < HCAP) atomicAdd(&h[dr], 1u);
-        }
-        bool pending = live && dr >= HCAP;
-        while (true) {
-            u64 act = ballot(pending);
-            if (!act) break;
-            int leader = __ffsll((long long)act) - 1;
-            u32 dl = __shfl(dr, leader, WAVE);
-            bool mine = pending && dr == dl;
-            u64 peers = ballot(mine);
-            if (lane_id() == leader) atomicAdd(&count[dl], (u32)__popcll(peers));
-            if (mine) pending = false;
         }
     }
     __syncthreads();
     for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK)
-        if (h[i]) atomicAdd(&count[i], h[i]);
+        if (h[i]) atomicAdd(&count[lo + i], h[i]);
 }
 
 // Names of a byte text's LMS substrings through the dictionary; same
@@ -600,20 +619,27 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
     d.zero(count, ((u64)K1 + 1) * sizeof(u32));
     {
         int pr = d.pbegin("dict_rank", 16.0 * m1);
-        // one workgroup per CU with the large histogram where the device
-        // gives a workgroup that much LDS, two with the small one
-        const bool big = d.lds_per_block >= DICT_HIST_BIG * 4 + 1024 && K1 > DICT_HIST_SMALL;
-        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), big ? 256 : 512);
-        auto go = [&](auto kern) {
-            kern<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E, ckey,
-                                                      crk, cbits, K1, SL, count, ctr);
-        };
-        if (aligned && big) go(k_dict_rank<true, DICT_HIST_BIG>);
-        else if (aligned) go(k_dict_rank<true, DICT_HIST_SMALL>);
-        else if (big) go(k_dict_rank<false, DICT_HIST_BIG>);
-        else go(k_dict_rank<false, DICT_HIST_SMALL>);
+        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), 512);
+        if (aligned)
+            k_dict_rank<true><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(
+                rk, lms_text, m1, m, D, Db, term, E, ckey, crk, cbits, K1, SL, count, ctr);
+        else
+            k_dict_rank<false><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(
+                rk, lms_text, m1, m, D, Db, term, E, ckey, crk, cbits, K1, SL, count, ctr);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
+        // the ranks above the first window, window by window from SL
+        for (u32 w = 1; (u64)w * DICT_HIST < K1; w++) {
+            const bool tail = w == DICT_WINDOWS - 1;
+            if (tail)
+                k_dict_count<true><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(SL, m1, w * DICT_HIST, K1,
+                                                                       count);
+            else
+                k_dict_count<false><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(SL, m1, w * DICT_HIST, K1,
+                                                                        count);
+            PSK_LAUNCH_CHECK();
+            if (tail) break;
+        }
     }
     // per rank: first slot (exclusive scan of counts), finished?, and the
     // dense id among the unfinished groups

@@ -21,7 +21,6 @@ struct Dev {
     u32 *pinned = nullptr;  // >= 4096 u32 of pinned host memory
     u32 *err = nullptr;     // device flag: an index left its range (internal error)
     Prof *prof = nullptr;
-    u32 lds_per_block = 65536;  // the device's LDS limit for one workgroup
 
     int pbegin(const char *name, double bytes) { return prof ? prof->begin(s, name, bytes) : -1; }
     void pend(int r) { if (prof) prof->end(r); }
