@@ -181,15 +181,17 @@ __device__ __forceinline__ u64 chain_window(u64 w, u32 r, u32 c, bool has_sb, u3
 // One pass measures the runs, one scan gives emission offsets, and one stable
 // multi-split by depth places them (two passes if a run exceeds 256).
 // Leaves every queue entry's window known (refilled where it had run out)
-// and its predecessor byte (0 if none) in dig[0..limit).  d_tmp: 2 u32.
+// and its predecessor byte in dig[0..limit), or `none` if it has no
+// predecessor: 0 for the L-scan, 255 for the S-scan, a byte that the scan
+// never induces from (0 is never > c, 255 never < c).  d_tmp: 2 u32.
 inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *dig, QPos qp,
-                         u32 c, const u32 *d_start, u64 limit, u32 *d_tmp) {
+                         u32 c, const u32 *d_start, u64 limit, u32 *d_tmp, u32 none) {
     const u64 a = d.read_u32(d_start);
     u32 *err = d.err;
     if (a > limit) fail(PSK_EDEVICE, "queue seed count exceeds its bucket part");
     Arena::Mark mk = d.a->mark();
     u32 *run = d.a->get<u32>(a + 1);
-    u8 *stop = d.a->get<u8>(a + 1);
+    unsigned short *stop = d.a->get<unsigned short>(a + 1);  // byte | has << 8
     u32 *maxr = d_tmp;
     d.zero(maxr, sizeof(u32));
     const u64 nchain = limit - a;
@@ -211,22 +213,24 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
                         PW[slot] = w;
                     }
                     const u32 v = Win::v(w);
-                    const u32 cp = v ? Win::b(w, 1) : 0u;
+                    const u32 cp = v ? Win::b(w, 1) : none;
                     dig[k] = (u8)cp;
                     u32 sb = 0;
                     if (v && cp == c) {
                         // the run of c's before x: inside the window, or
-                        // beyond it from the decided bits
+                        // beyond it from the decided bits; sb = the byte
+                        // before the run (bit 8: there is one)
                         r = 1;
                         while (r < v && Win::b(w, r + 1) == c) r++;
-                        if (r < v) sb = Win::b(w, r + 1);
-                        else if (!Win::e(w)) {
+                        if (r < v) {
+                            sb = Win::b(w, r + 1) | 256u;
+                        } else if (!Win::e(w)) {
                             r = pr.run_before(x);
-                            sb = pr.byte(x - r);
+                            if (pr.has(x - r)) sb = pr.byte(x - r) | 256u;
                         }
                     }
                     run[k] = r;
-                    stop[k] = (u8)sb;
+                    stop[k] = (unsigned short)sb;
                 }
                 // same-address atomics serialise in L2: only raise the maximum
                 // when this wave beats the value already stored
@@ -259,12 +263,13 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
         const u64 slot = qp(k);
         u32 x = SA[slot], o = eo[k];
         u64 w = PW[slot];
-        u32 sb = stop[k];
-        bool hs = pr.has(x - r);
+        const u32 st = stop[k];
+        const bool hs = st >> 8;
+        const u32 sb = st & 255u;
         for (u32 t = 1; t <= r; t++) {
             ek[o + t - 1] = t - 1;
             ev[o + t - 1] = x - t;
-            eb[o + t - 1] = t < r ? (u8)c : (u8)sb;
+            eb[o + t - 1] = t < r ? (u8)c : (u8)(hs ? sb : none);
             ew[o + t - 1] = chain_window(w, r, c, hs, sb, t);
         }
     });
@@ -356,7 +361,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], nlms = B.nLMS[c];
         const u32 cc = (u32)c;
-        if (nl) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp);
+        if (nl) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp, 0u);
         const u64 R = nl + nlms;
         if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
@@ -402,7 +407,9 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
         const u32 cc = (u32)c;
-        if (ns) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp);
+        if (ns)
+            chain_queue8(d, pr, aligned, SA, PW, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp,
+                         255u);
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };
@@ -411,9 +418,9 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
             [=] __device__(u64 j, u32 *key) {
                 u32 k;
                 if (j < ns) {
-                    u32 x = SA[pos(j)];
+                    // cached predecessor byte, 255 if none
                     u32 cp = dig[j];
-                    k = (pr.has(x) && cp < cc) ? cp : 255u;
+                    k = cp < cc ? cp : 255u;
                 } else {
                     // an L entry: its window is known since its bucket's L-scan
                     u64 w = PW[pos(j)];
