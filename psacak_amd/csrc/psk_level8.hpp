@@ -250,20 +250,32 @@ inline void batch8_run(Runtime &rt, const u8 *T, const u64 *off, u64 K, u32 *SA)
     });
     BatchInfo bi{dends, hends.size(), EB, SB};
     psacak8_run(rt, T, N, SA, &bi);
-    // stable partition of the concatenation's SA by text id
-    u64 *k0 = d.a->get<u64>(N), *k1 = d.a->get<u64>(N);
-    u32 *v0 = d.a->get<u32>(N), *v1 = d.a->get<u32>(N);
-    for_each(d, N, [=] __device__(u64 j) {
-        u32 x = SA[j];
-        u64 lo = 0, hi = KK;  // last i with doff[i] <= x, among non-empty texts
+    // stable partition of the concatenation's SA by text id.  The text of a
+    // position x is the last i with doff[i] <= x (empty texts share their
+    // offset with the next text); a table of the text at every 4 KiB of
+    // positions narrows the search to the texts that start in x's block.
+    constexpr int BTS = 12;
+    const u64 nb = (N + (1ull << BTS) - 1) >> BTS;
+    u32 *bt = d.a->get<u32>(nb + 1);
+    auto last_le = [=] __device__(u64 lo, u64 hi, u64 x) -> u64 {
+        // largest i in [lo, hi) with doff[i] <= x, given doff[lo] <= x
         while (hi - lo > 1) {
             u64 mid = (lo + hi) / 2;
             if (doff[mid] <= x) lo = mid;
             else hi = mid;
         }
-        while (lo + 1 < KK && doff[lo + 1] <= x) lo++;  // skip empty texts ending at x
-        k0[j] = lo;
-        v0[j] = x - doff[lo];
+        return lo;
+    };
+    for_each(d, nb, [=] __device__(u64 b) { bt[b] = (u32)last_le(0, KK, b << BTS); });
+    u64 *k0 = d.a->get<u64>(N), *k1 = d.a->get<u64>(N);
+    u32 *v0 = d.a->get<u32>(N), *v1 = d.a->get<u32>(N);
+    for_each(d, N, [=] __device__(u64 j) {
+        const u32 x = SA[j];
+        const u64 b = x >> BTS;
+        const u64 hi = b + 1 < nb ? (u64)bt[b + 1] + 1 : KK;
+        const u64 i = last_le(bt[b], hi < KK ? hi : KK, x);
+        k0[j] = i;
+        v0[j] = x - doff[i];
     });
     bool sw = radix_sort_kv(d, N, k0, v0, k1, v1, bit_width(K > 1 ? K - 1 : 0));
     PSK_HIP(hipMemcpyAsync(SA, sw ? v1 : v0, N * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
