@@ -317,7 +317,7 @@ def bench_served(args):
                 if dt > 0:
                     time.sleep(dt)
                 t_sub[i] = time.perf_counter()
-                ids[i] = srv.submit(texts[i])
+                ids[i] = srv.submit_into(texts[i])  # text borrowed, SA written in place
 
         log("serving")
         t0 = time.perf_counter()

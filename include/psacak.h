@@ -154,6 +154,13 @@ uint64_t psk_server_footprint(uint64_t bytes, int batched);
 /* Submit a text (copied).  *id receives the request id.  PSK_EBUDGET if the
  * request alone exceeds the budget. */
 int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *id);
+/* As psk_server_submit, but the text is not copied and the SA is written
+ * straight into sa (n entries): both buffers must stay valid and untouched
+ * until poll/wait reports the request done (poll/wait then copy nothing).
+ * No counterpart in the crate (whose psacak_inplace, lib.rs:77-82, is the
+ * blocking form of the same contract). */
+int psk_server_submit_into(psk_server *s, const uint8_t *text, uint64_t n, uint32_t *sa,
+                           uint64_t *id);
 /* *done = 1 and, when finished, the SA (n entries) is copied to sa and
  * *latency_ms = submit-to-completion time; the request is then released.
  * Returns the request's status (or PSK_ENOTFOUND). */

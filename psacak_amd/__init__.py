@@ -86,6 +86,7 @@ def _load():
         "psk_server_create": ([ctypes.POINTER(vp), u64, i32], i32),
         "psk_server_footprint": ([u64, i32], u64),
         "psk_server_submit": ([vp, vp, u64, ctypes.POINTER(u64)], i32),
+        "psk_server_submit_into": ([vp, vp, u64, vp, ctypes.POINTER(u64)], i32),
         "psk_server_poll": ([vp, u64, ctypes.POINTER(i32), vp, ctypes.POINTER(ctypes.c_double)], i32),
         "psk_server_wait": ([vp, u64, vp, ctypes.POINTER(ctypes.c_double), ctypes.c_double], i32),
         "psk_server_peak_bytes": ([vp], u64),
@@ -377,31 +378,53 @@ class Server:
         return int(_lib.psk_server_footprint(nbytes, 1 if batched else 0))
 
     def submit(self, text) -> int:
+        """Queue a copy of text; the SA is copied out by poll/wait."""
         t = _as_u8(text)
         rid = ctypes.c_uint64()
         rc = _lib.psk_server_submit(self._h, t.ctypes.data, len(t), ctypes.byref(rid))
         _raise(rc)
-        self._n[rid.value] = len(t)
+        self._n[rid.value] = (len(t), None, None)
         return rid.value
+
+    def submit_into(self, text, out=None) -> int:
+        """Queue text without copying it; the SA is written straight into out
+        (a new uint32 array if None), which poll/wait return.  The server keeps
+        references to both until the request is done."""
+        t = _as_u8(text)
+        if out is None:
+            out = np.empty(len(t), dtype=np.uint32)
+        if out.dtype != np.uint32 or not out.flags.c_contiguous or len(out) < len(t):
+            raise ValueError("out must be a contiguous uint32 array of at least len(text)")
+        rid = ctypes.c_uint64()
+        rc = _lib.psk_server_submit_into(self._h, t.ctypes.data, len(t), out.ctypes.data,
+                                         ctypes.byref(rid))
+        _raise(rc)
+        self._n[rid.value] = (len(t), t, out)
+        return rid.value
+
+    def _out(self, rid):
+        n, _, out = self._n.get(rid, (0, None, None))
+        return (out if out is not None else np.empty(n, dtype=np.uint32)), out is not None
 
     def poll(self, rid: int):
         """(done, sa or None, latency_ms or None); a finished request is released."""
         done = ctypes.c_int(0)
         lat = ctypes.c_double(0)
-        sa = np.empty(self._n.get(rid, 0), dtype=np.uint32)
-        rc = _lib.psk_server_poll(self._h, rid, ctypes.byref(done), sa.ctypes.data,
-                                  ctypes.byref(lat))
+        sa, borrowed = self._out(rid)
+        rc = _lib.psk_server_poll(self._h, rid, ctypes.byref(done),
+                                  None if borrowed else sa.ctypes.data, ctypes.byref(lat))
         if not done.value and rc == PSK_OK:
             return False, None, None
         self._n.pop(rid, None)
         _raise(rc)
-        return True, sa, lat.value
+        return True, sa[: len(sa)], lat.value
 
     def wait(self, rid: int, timeout_ms: float = -1.0):
         """Block until the request finishes; returns (sa, latency_ms)."""
         lat = ctypes.c_double(0)
-        sa = np.empty(self._n.get(rid, 0), dtype=np.uint32)
-        rc = _lib.psk_server_wait(self._h, rid, sa.ctypes.data, ctypes.byref(lat), timeout_ms)
+        sa, borrowed = self._out(rid)
+        rc = _lib.psk_server_wait(self._h, rid, None if borrowed else sa.ctypes.data,
+                                  ctypes.byref(lat), timeout_ms)
         if rc != PSK_EBUSY:
             self._n.pop(rid, None)
         _raise(rc)

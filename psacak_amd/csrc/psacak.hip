@@ -9,6 +9,7 @@
 
 #include "psk_level8.hpp"
 #include "psk_server.hpp"
+#include "psk_xfer.hpp"
 
 using namespace psk;
 
@@ -23,6 +24,7 @@ struct psk_ctx {
     std::string err;
     Runtime rt;
     Prof prof;
+    Xfer xfer;
     double last_ms[4] = {0, 0, 0, 0};
 };
 
@@ -199,6 +201,7 @@ void psk_ctx_destroy(psk_ctx *ctx) {
     if (ctx->ready) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->arena.release();
+        ctx->xfer.release();
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);
@@ -262,10 +265,9 @@ int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *
         try {
             u8 *dt = c->arena.get<u8>(n);
             u32 *ds = c->arena.get<u32>(n);
-            PSK_HIP(hipMemcpyAsync(dt, text, n, hipMemcpyHostToDevice, c->stream));
+            upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n}}, dt);
             run8_dev(c, dt, n, ds);
-            PSK_HIP(hipMemcpyAsync(sa, ds, n * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
-            PSK_HIP(hipStreamSynchronize(c->stream));
+            download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
         } catch (...) {
             c->arena.reset(mk);
             throw;
@@ -289,10 +291,9 @@ static int host32(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa, 
         try {
             u32 *dt = c->arena.get<u32>(n);
             u32 *ds = c->arena.get<u32>(n);
-            PSK_HIP(hipMemcpyAsync(dt, text, n * sizeof(u32), hipMemcpyHostToDevice, c->stream));
+            upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n * sizeof(u32)}}, dt);
             run32_dev(c, dt, n, ds, raw);
-            PSK_HIP(hipMemcpyAsync(sa, ds, n * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
-            PSK_HIP(hipStreamSynchronize(c->stream));
+            download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
         } catch (...) {
             c->arena.reset(mk);
             throw;
@@ -431,16 +432,15 @@ int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
             try {
                 u8 *dt = c->arena.get<u8>(N);
                 u32 *ds = c->arena.get<u32>(N);
-                std::vector<u8> stage(N);
-                for (u64 i = 0; i < K; i++) memcpy(stage.data() + off[i], texts[g.first + i], lens[g.first + i]);
-                c->rt.d.upload(dt, stage.data(), N);
+                std::vector<Seg> tin, sout;
+                for (u64 i = 0; i < K; i++) {
+                    tin.push_back(Seg{(char *)texts[g.first + i], lens[g.first + i]});
+                    sout.push_back(Seg{(char *)sas[g.first + i], lens[g.first + i] * sizeof(u32)});
+                }
+                upload_segs(c->rt.d, c->xfer, tin, dt);
                 if (K == 1) psacak8_run(c->rt, dt, N, ds);
                 else batch8_run(c->rt, dt, off.data(), K, ds);
-                std::vector<u32> out(N);
-                PSK_HIP(hipMemcpyAsync(out.data(), ds, N * sizeof(u32), hipMemcpyDeviceToHost, c->stream));
-                PSK_HIP(hipStreamSynchronize(c->stream));
-                for (u64 i = 0; i < K; i++)
-                    memcpy(sas[g.first + i], out.data() + off[i], lens[g.first + i] * sizeof(u32));
+                download_segs(c->rt.d, c->xfer, ds, sout);
             } catch (...) {
                 c->arena.reset(mk);
                 throw;
@@ -554,13 +554,39 @@ static void server_worker(psk_server *s, int w) {
                         }
                     }
                     u64 f = psk_server_footprint(tot, cand.size() > 1);
-                    if (s->budget == 0 || s->inflight == 0 || s->inflight + f <= s->budget) {
+                    // device memory the other workers hold: their arenas, or
+                    // their running job's footprint if larger; idle_cache: what
+                    // idle workers keep cached
+                    u64 others = 0, idle_cache = 0;
+                    for (int v = 0; v < s->nworkers; v++) {
+                        if (v == w) continue;
+                        const u64 r = s->ctxs[v]->arena.reserved();
+                        others += std::max(r, s->wfoot[v]);
+                        if (!s->wfoot[v]) idle_cache += r;
+                    }
+                    const u64 mine = c->arena.reserved();
+                    bool admit = s->budget == 0 || others + std::max(mine, f) <= s->budget;
+                    if (!admit && mine && others + f <= s->budget) {
+                        // this worker's cached chunks are what stands in the way
+                        (void)psk_ctx_trim(c);
+                        admit = true;
+                    }
+                    if (admit) {
                         for (size_t i = 0; i < cand.size(); i++) s->pending.pop_front();
                         s->inflight += f;
+                        s->wfoot[w] = f;
                         foot = f;
                         job.swap(cand);
                         break;
                     }
+                    if (mine) {
+                        // idle and blocked: give back the cache, let the others retry
+                        (void)psk_ctx_trim(c);
+                        s->cv_work.notify_all();
+                        continue;
+                    }
+                    // idle workers' caches can be given back: wake them
+                    if (idle_cache) s->cv_work.notify_all();
                 }
                 s->cv_work.wait(lk);
             }
@@ -568,15 +594,15 @@ static void server_worker(psk_server *s, int w) {
         int rc;
         if (job.size() == 1) {
             SrvReq *r = job[0];
-            rc = psk_psacak_inplace(c, r->text.data(), r->n, r->sa.data(), r->n);
+            rc = psk_psacak_inplace(c, r->tp, r->n, r->sp, r->n);
         } else {
             std::vector<const uint8_t *> tp;
             std::vector<uint64_t> lp;
             std::vector<uint32_t *> sp;
             for (SrvReq *r : job) {
-                tp.push_back(r->text.data());
+                tp.push_back(r->tp);
                 lp.push_back(r->n);
-                sp.push_back(r->sa.data());
+                sp.push_back(r->sp);
             }
             rc = psk_batch8(c, tp.data(), lp.data(), sp.data(), job.size());
         }
@@ -590,10 +616,14 @@ static void server_worker(psk_server *s, int w) {
                 std::vector<u8>().swap(r->text);
             }
             s->inflight -= foot;
+            s->wfoot[w] = 0;
             u64 res = 0;
             for (psk_ctx *x : s->ctxs) res += x->arena.reserved();
             s->peak = std::max(s->peak, res);
-            if (s->budget && res > s->budget) (void)psk_ctx_trim(c);
+            // cached chunks of finished jobs are not in flight: a worker keeps
+            // at most its share of the budget between jobs
+            if (s->budget && (res > s->budget || c->arena.reserved() > s->budget / s->nworkers))
+                (void)psk_ctx_trim(c);
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
@@ -616,23 +646,32 @@ int psk_server_create(psk_server **out, uint64_t hbm_budget_bytes, int max_worke
         }
         s->ctxs.push_back(c);
     }
+    s->wfoot.assign(max_workers, 0);
     for (int w = 0; w < max_workers; w++) s->workers.emplace_back(server_worker, s, w);
     *out = s;
     return PSK_OK;
 }
 
-int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *id) {
+static int submit_req(psk_server *s, const uint8_t *text, uint64_t n, uint32_t *sa, uint64_t *id) {
     if (!s || !id || (n && !text)) return PSK_EINVAL;
     if (n >= 0xFFFFFFFFull) return PSK_ETOOLONG;
     u64 f = psk_server_footprint(n, 0);
     if (s->budget && f > s->budget) return PSK_EBUDGET;
     std::unique_ptr<SrvReq> r(new (std::nothrow) SrvReq());
     if (!r) return PSK_ENOMEM;
-    try {
-        r->text.assign(text, text + n);
-        r->sa.resize(n);
-    } catch (...) {
-        return PSK_ENOMEM;
+    if (sa) {
+        r->tp = text;
+        r->sp = sa;
+        r->owned = false;
+    } else {
+        try {
+            r->text.assign(text, text + n);
+            r->sa.resize(n);
+        } catch (...) {
+            return PSK_ENOMEM;
+        }
+        r->tp = r->text.data();
+        r->sp = r->sa.data();
     }
     r->n = n;
     r->t_submit = now_ms();
@@ -647,8 +686,18 @@ int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *
     return PSK_OK;
 }
 
+int psk_server_submit(psk_server *s, const uint8_t *text, uint64_t n, uint64_t *id) {
+    return submit_req(s, text, n, nullptr, id);
+}
+
+int psk_server_submit_into(psk_server *s, const uint8_t *text, uint64_t n, uint32_t *sa,
+                           uint64_t *id) {
+    if (n && !sa) return PSK_EINVAL;
+    return submit_req(s, text, n, n ? sa : nullptr, id);
+}
+
 static int take_result(psk_server *s, std::unique_ptr<SrvReq> &r, uint32_t *sa, double *lat) {
-    if (sa && r->n) memcpy(sa, r->sa.data(), r->n * sizeof(u32));
+    if (sa && r->owned && r->n) memcpy(sa, r->sa.data(), r->n * sizeof(u32));
     if (lat) *lat = r->t_done - r->t_submit;
     int st = r->status;
     s->reqs.erase(r->id);

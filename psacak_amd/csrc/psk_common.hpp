@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -122,7 +123,8 @@ class Arena {
     ~Arena() { release(); }
 
     void set_budget(u64 b) { budget_ = b; }
-    u64 reserved() const { return reserved_; }
+    // readable from other threads (a server sums its workers' arenas)
+    u64 reserved() const { return reserved_.load(std::memory_order_relaxed); }
     u64 peak() const { return peak_; }
 
     Mark mark() const { return {cur_, top_}; }
@@ -141,9 +143,10 @@ class Arena {
             top_ = 0;
         }
         u64 cap = std::max<u64>(bytes, min_chunk_);
-        if (budget_ && reserved_ + cap > budget_) {
+        const u64 res = reserved();
+        if (budget_ && res + cap > budget_) {
             cap = bytes;
-            if (reserved_ + cap > budget_)
+            if (res + cap > budget_)
                 fail(PSK_EBUDGET, "device workspace exceeds the HBM budget");
         }
         char *p = nullptr;
@@ -152,8 +155,8 @@ class Arena {
             fail(PSK_ENOMEM, "hipMalloc of " + std::to_string(cap) + " bytes failed");
         }
         chunks_.push_back({p, cap});
-        reserved_ += cap;
-        peak_ = std::max(peak_, reserved_);
+        reserved_.store(res + cap, std::memory_order_relaxed);
+        peak_ = std::max(peak_, res + cap);
         cur_ = chunks_.size() - 1;
         top_ = bytes;
         return p;
@@ -165,7 +168,7 @@ class Arena {
     void release() {
         for (auto &c : chunks_) (void)hipFree(c.p);
         chunks_.clear();
-        reserved_ = 0;
+        reserved_.store(0, std::memory_order_relaxed);
         cur_ = top_ = 0;
     }
 
@@ -174,7 +177,8 @@ class Arena {
     std::vector<Chunk> chunks_;
     size_t cur_ = 0;
     u64 top_ = 0;
-    u64 reserved_ = 0, peak_ = 0, budget_ = 0;
+    std::atomic<u64> reserved_{0};
+    u64 peak_ = 0, budget_ = 0;
     u64 min_chunk_ = 64ull << 20;
 };
 

@@ -31,8 +31,11 @@ inline double now_ms() {
 
 struct SrvReq {
     u64 id = 0, n = 0, foot = 0;
-    std::vector<u8> text;
+    std::vector<u8> text;  // owned copies (psk_server_submit)
     std::vector<u32> sa;
+    const u8 *tp = nullptr;  // the text and SA buffers the worker uses: the
+    u32 *sp = nullptr;       // owned copies, or the caller's (submit_into)
+    bool owned = true;
     double t_submit = 0, t_done = 0;
     int status = PSK_OK;
     bool done = false;
@@ -51,8 +54,12 @@ struct psk_server {
     bool stop = false;
     std::vector<std::thread> workers;
     std::vector<psk_ctx *> ctxs;
+    std::vector<psk::u64> wfoot;  // footprint of each worker's running job (0: idle)
     // requests up to this size are coalesced; a coalesced batch holds at
     // most kBatchBytes of text
     static constexpr psk::u64 kSmall = 16ull << 20;
-    static constexpr psk::u64 kBatchBytes = 512ull << 20;
+#ifndef PSK_SRV_BATCH_MB
+#define PSK_SRV_BATCH_MB 512
+#endif
+    static constexpr psk::u64 kBatchBytes = (psk::u64)PSK_SRV_BATCH_MB << 20;
 };

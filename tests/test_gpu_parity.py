@@ -298,6 +298,42 @@ def test_batch_text_ends(dict_naming, rounds):
         assert np.array_equal(o, oracle.sacak8(np.frombuffer(t, dtype=np.uint8))), i
 
 
+def test_host_staged_transfers(ctx):
+    # host buffers of several staging chunks (64 MiB) with odd sizes: the
+    # host entry points return exactly what the device entry points compute
+    import torch
+    note("host staged transfers u8")
+    t = gen.markov_dna((24 << 20) + 5, seed=12)
+    got = P.psacak(t, ctx)
+    dt = torch.from_numpy(t).to("cuda:0")
+    ds = torch.empty(len(t), dtype=torch.int32, device="cuda:0")
+    P.psacak8_dev(dt, ds, ctx=ctx)
+    torch.cuda.synchronize()
+    assert np.array_equal(got, ds.cpu().numpy().view(np.uint32))
+    assert P.check8_dev(dt, ds, len(t), ctx=ctx)
+    note("host staged transfers u32")
+    z = gen.zipf32((20 << 20) + 3, sigma=1 << 16, s=1.1, repeat_frac=0.1, seed=5)
+    g32 = P.psacak32_raw(z, ctx)
+    dz = torch.from_numpy(z.view(np.int32)).to("cuda:0")
+    dzs = torch.empty(len(z), dtype=torch.int32, device="cuda:0")
+    P.psacak32_raw_dev(dz, dzs, len(z), ctx=ctx)
+    torch.cuda.synchronize()
+    assert np.array_equal(g32, dzs.cpu().numpy().view(np.uint32))
+    note("host staged transfers batch")
+    L = gen.lengths(60, 1 << 10, 3 << 20, seed=19)
+    texts = [gen.iid_dna(int(n), seed=300 + i) for i, n in enumerate(L)]
+    outs = P.batch8(texts, ctx)
+    off = np.zeros(len(texts) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(x) for x in texts])
+    dt = torch.from_numpy(np.concatenate(texts)).to("cuda:0")
+    ds = torch.empty(int(off[-1]), dtype=torch.int32, device="cuda:0")
+    P.batch8_dev(dt, off, ds, ctx=ctx)
+    torch.cuda.synchronize()
+    sa = ds.cpu().numpy().view(np.uint32)
+    for i, o in enumerate(outs):
+        assert np.array_equal(o, sa[off[i]:off[i + 1]]), i
+
+
 def test_batch_matches_single_calls_on_repeats(ctx):
     note("batch8 identical texts")
     t = gen.markov_dna(50000, seed=5)
@@ -321,8 +357,9 @@ def test_server_results_equal_alone_and_in_the_mix(ctx):
         ids = {}
 
         def producer():
+            # copying and borrowing submissions, interleaved
             for i, t in enumerate(texts):
-                ids[i] = srv.submit(t)
+                ids[i] = srv.submit(t) if i % 2 else srv.submit_into(t)
                 time.sleep(float(rng.exponential(0.0005)))
 
         th = threading.Thread(target=producer)
