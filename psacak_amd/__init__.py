@@ -29,7 +29,9 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libpsacak_amd.so")
+# PSACAK_AMD_LIB selects another build of the library (the autotuner times
+# candidate builds this way); by default the in-tree build
+_LIB_PATH = os.environ.get("PSACAK_AMD_LIB") or os.path.join(_HERE, "libpsacak_amd.so")
 
 PSK_OK, PSK_EINVAL, PSK_ETOOLONG, PSK_EPRECOND = 0, -1, -2, -3
 PSK_ENOMEM, PSK_EBUDGET, PSK_EDEVICE, PSK_EBUSY, PSK_ENOTFOUND = -4, -5, -6, -7, -8

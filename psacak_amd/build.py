@@ -38,9 +38,24 @@ def _run(cmd):
     return out
 
 
-def build_hip(verbose: bool = False, out_path: str = LIB, defines=()) -> str:
-    """Compile the library; defines (e.g. "PSK_RS_ROUNDS=16") override tile
-    constants for tuning builds."""
+TUNING = os.path.join(HERE, "tuning.json")
+
+
+def tuned_defines():
+    """Tile constants chosen by the autotuner (psacak_amd/tune.py), if any."""
+    if not os.path.exists(TUNING):
+        return []
+    import json
+
+    with open(TUNING) as f:
+        return [f"{k}={int(v)}" for k, v in json.load(f).get("defines", {}).items()]
+
+
+def build_hip(verbose: bool = False, out_path: str = LIB, defines=None) -> str:
+    """Compile the library.  defines (e.g. "PSK_RS_ROUNDS=16") override tile
+    constants; by default the autotuner's choice (tuning.json) applies."""
+    if defines is None:
+        defines = tuned_defines()
     tmp = out_path + ".tmp"
     out = _run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-o", tmp,
                 os.path.join(CSRC, "psacak.hip")])

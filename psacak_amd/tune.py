@@ -1,0 +1,123 @@
+"""Autotuner for the tile shapes of psacak_amd (run on the GPU machine).
+
+The kernels' tile and workgroup shapes are build-time constants:
+
+* ``PSK_RS_ROUNDS`` -- rounds of 256 pairs per radix-sort tile (tile = 256 x rounds);
+* ``PSK_RS_CT``     -- radix-count tiles per workgroup;
+* ``PSK_MS_ROUNDS`` -- rounds of 256 items per multi-split tile (level-0 induce).
+
+The tuner searches them one coordinate at a time: for each candidate value it
+builds the library with hipcc (``build.build_hip(defines=...)``) into a
+scratch directory, times a real psacak8 call on a seeded DNA text in a fresh
+process that loads that build (``PSACAK_AMD_LIB``), checks the SA there with
+the device checker, and keeps the fastest.  The result is written to
+``psacak_amd/tuning.json``, which ``build.py`` applies to every later build.
+
+    python -m psacak_amd.tune [--n BYTES] [--reps K] [--out FILE]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+TUNING = os.path.join(HERE, "tuning.json")
+
+SPACE = {
+    "PSK_RS_ROUNDS": [8, 4, 16],
+    "PSK_RS_CT": [1, 2, 4],
+    "PSK_MS_ROUNDS": [16, 8, 32],
+}
+
+# Timed in a child process: the library under test is chosen by PSACAK_AMD_LIB
+# before the package is imported.
+_CHILD = r"""
+import json, sys, time
+sys.path.insert(0, {root!r})
+import numpy as np, torch
+import psacak_amd as P
+from psacak_amd import gen
+n, reps = {n}, {reps}
+t = gen.markov_dna(n, seed=20260101, repeat_frac=0.10)
+dt = torch.from_numpy(t).to("cuda:0")
+ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+ctx = P.Context()
+P.psacak8_dev(dt, ds, n, ctx=ctx)
+torch.cuda.synchronize()
+ms = []
+for _ in range(reps):
+    P.psacak8_dev(dt, ds, n, ctx=ctx)
+    ms.append(float(ctx.last_timing_ms()[0]))
+ok = P.check8_dev(dt, ds, n, ctx=ctx)
+print(json.dumps({{"ms": min(ms), "all_ms": ms, "ok": bool(ok)}}))
+"""
+
+
+def _measure(lib: str, n: int, reps: int) -> dict:
+    env = dict(os.environ, PSACAK_AMD_LIB=lib)
+    code = _CHILD.format(root=ROOT, n=n, reps=reps)
+    r = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"timing run failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def tune(n: int, reps: int, out: str, log=print) -> dict:
+    from psacak_amd import build
+
+    best = {k: v[0] for k, v in SPACE.items()}
+    tried = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        def run(cfg):
+            key = tuple(sorted(cfg.items()))
+            if key in tried:
+                return tried[key]
+            lib = os.path.join(tmp, f"lib{len(tried)}.so")
+            t0 = time.time()
+            build.build_hip(out_path=lib, defines=[f"{k}={v}" for k, v in cfg.items()])
+            res = _measure(lib, n, reps)
+            if not res["ok"]:
+                raise RuntimeError(f"candidate {cfg} produced an invalid SA")
+            tried[key] = res["ms"]
+            log(f"[tune] {cfg}: {res['ms']:.3f} ms (build+run {time.time() - t0:.0f} s)")
+            return res["ms"]
+
+        cur = run(best)
+        for name, values in SPACE.items():
+            for v in values:
+                if v == best[name]:
+                    continue
+                cfg = dict(best, **{name: v})
+                ms = run(cfg)
+                if ms < cur * 0.99:  # keep a change only for a clear gain
+                    best, cur = cfg, ms
+    rec = {
+        "defines": best,
+        "ms": cur,
+        "workload": f"psacak8 on {n} bytes of seeded Markov DNA (bench generator), best of {reps}",
+        "candidates": [{"defines": dict(k), "ms": v} for k, v in tried.items()],
+    }
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+    log(f"[tune] best {best}: {cur:.3f} ms -> {out}")
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--n", type=int, default=1 << 28, help="text bytes of the timed call")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=TUNING)
+    a = ap.parse_args()
+    tune(a.n, a.reps, a.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -131,3 +131,16 @@ def test_server_reports_device_error_without_gpu():
         with pytest.raises(P.PskError) as e:
             s.wait(rid, timeout_ms=30000)
         assert e.value.code == P.PSK_EDEVICE
+
+
+def test_tuned_defines_read_from_tuning_file(tmp_path, monkeypatch):
+    import json
+
+    from psacak_amd import build
+
+    monkeypatch.setattr(build, "TUNING", str(tmp_path / "none.json"))
+    assert build.tuned_defines() == []
+    f = tmp_path / "tuning.json"
+    f.write_text(json.dumps({"defines": {"PSK_RS_ROUNDS": 8, "PSK_MS_ROUNDS": 16}}))
+    monkeypatch.setattr(build, "TUNING", str(f))
+    assert sorted(build.tuned_defines()) == ["PSK_MS_ROUNDS=16", "PSK_RS_ROUNDS=8"]
