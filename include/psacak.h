@@ -71,6 +71,15 @@ int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds);
  * way.  No counterpart in the crate; results are identical.  PSK_EINVAL
  * unless on is 0 or 1. */
 int psk_ctx_set_dict_naming(psk_ctx *ctx, int on);
+/* A recursion level whose text has at least distinct_per_1024/1024 distinct
+ * symbols is sorted by prefix doubling over all its suffixes instead of
+ * naming and recursing further (no counterpart in the crate, which always
+ * recurses, psacak32.rs:24-56; results are identical).  A level whose
+ * suffixes keep tying (long repeats: ties that stop shrinking after 2 rounds,
+ * or more than an eighth tied after 4) recurses after all, and so do the
+ * levels below it.  0 = always recurse.  Default 128.  PSK_EINVAL outside
+ * [0, 1024]. */
+int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024);
 /* Device bytes a psacak8 call on a text of n bytes needs beyond its
  * input and output buffers (the admission footprint). */
 uint64_t psk_workspace_bytes8(uint64_t n);

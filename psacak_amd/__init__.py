@@ -70,6 +70,7 @@ def _load():
         "psk_ctx_trim": ([vp], i32),
         "psk_ctx_set_direct_rounds": ([vp, i32], i32),
         "psk_ctx_set_dict_naming": ([vp, i32], i32),
+        "psk_ctx_set_doubling": ([vp, i32], i32),
         "psk_workspace_bytes8": ([u64], u64),
         "psk_workspace_bytes32": ([u64], u64),
         "psk_psacak_inplace": ([vp, vp, u64, vp, u64], i32),
@@ -182,6 +183,12 @@ class Context:
         """Rounds of direct LMS-substring sorting before induced sorting takes
         over; 0 = always induced sorting.  Results are identical either way."""
         self._check(_lib.psk_ctx_set_direct_rounds(self._h, int(rounds)))
+
+    def set_doubling(self, distinct_per_1024: int):
+        """Recursion levels whose text has at least distinct_per_1024/1024
+        distinct symbols are sorted by prefix doubling (0 = always recurse).
+        Results are identical either way."""
+        self._check(_lib.psk_ctx_set_doubling(self._h, int(distinct_per_1024)))
 
     def set_dict_naming(self, on: bool):
         """Round 1 of direct naming at level 0 through a dictionary of the

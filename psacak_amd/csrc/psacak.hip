@@ -220,6 +220,12 @@ int psk_ctx_set_direct_rounds(psk_ctx *ctx, int rounds) {
     return PSK_OK;
 }
 
+int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024) {
+    if (!ctx || distinct_per_1024 < 0 || distinct_per_1024 > 1024) return PSK_EINVAL;
+    ctx->rt.doubling_distinct = (u32)distinct_per_1024;
+    return PSK_OK;
+}
+
 int psk_ctx_set_dict_naming(psk_ctx *ctx, int on) {
     if (!ctx || on < 0 || on > 1) return PSK_EINVAL;
     ctx->rt.dict_naming = on != 0;

@@ -40,6 +40,11 @@ struct Runtime {
     // rounds of direct LMS-substring sorting before induced sorting takes
     // over (long repeated substrings); 0 = always induced sorting
     int direct_rounds = 32;
+    // a recursion level whose text has at least this fraction of distinct
+    // symbols (in 1/1024) is sorted by prefix doubling instead of recursing;
+    // 0 = always recurse
+    u32 doubling_distinct = 128;
+    bool doubling_gave_up = false;  // in this call (reset by each entry point)
     // round 1 of direct naming at level 0 through a dictionary of the
     // distinct keys (false: by sorting every key)
     bool dict_naming = true;
@@ -82,11 +87,17 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 // tails (with rank): number of symbols of rank <= c, so the bucket of rank c
 // is [tails[c-1], tails[c]) and the initial labels are its head + 1 (L) or
 // its tail (S, last-slot labels).
-inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
+// giveup: return false (RK partly relabelled) when the ties look like long
+// repeats: after round 2 still more than a sixteenth of the items tied and
+// fewer than a quarter of round 1's ties resolved, or after round 4 more than
+// an eighth tied.
+inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        bool last_slot_labels, const u32 *rank = nullptr,
-                       const u32 *tails = nullptr, u32 k = 0) {
+                       const u32 *tails = nullptr, u32 k = 0, bool giveup = false) {
     Dev &d = rt.d;
-    if (A == 0) return;
+    if (A == 0) return true;
+    const u32 A0 = A;
+    u32 Aprev = A;
     Arena::Mark mk = d.a->mark();
     u32 *act = d.a->get<u32>(A);
     u32 *vt = d.a->get<u32>(A);
@@ -164,9 +175,17 @@ inline void chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
         h *= 2;
         if (h > m + 1 && A > 0) fail(PSK_EDEVICE, "chain sort did not converge");
+        if (giveup && A > 0 &&
+            ((rounds == 2 && A > A0 / 16 && (u64)A * 4 > (u64)Aprev * 3) ||
+             (rounds == 4 && A > A0 / 8))) {
+            d.a->reset(mk);
+            return false;
+        }
+        Aprev = A;
     }
     rt.max_rounds = std::max(rt.max_rounds, rounds);
     d.a->reset(mk);
+    return true;
 }
 
 // L and S labels for a scan of a bucket-pointer text s[0..m).
@@ -331,6 +350,102 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
 // map (optional): SA entries are written as map[x] (see write_sa_from_labels).
 // rank, tails (optional): dense rank of every symbol of s (k distinct
 // values) and the cumulative symbol counts per rank.
+// Long exact repeats in a level's text (symbols rank[0..m)): the share of
+// 32-symbol windows that occur more than once, estimated on content-defined
+// anchors (positions whose 8-symbol hash is 0 mod 16, so the copies of a
+// repeat are anchored alike).  Anchor hashes are sorted and neighbours
+// compared.  Returns duplicates / anchors (0 when there are none).
+constexpr int REP_WIN = 32;
+__device__ __forceinline__ u32 rep_hash8(const u32 *rank, u64 i) {
+    u32 h = 0x243F6A88u;
+    #pragma unroll
+    for (int j = 0; j < 8; j++) h = (h ^ rank[i + j]) * 0x9E3779B1u;
+    return h ^ (h >> 15);
+}
+inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
+    if (m < 4 * (u64)REP_WIN) return 0.0;
+    Arena::Mark mk = d.a->mark();
+    const u64 np = m - REP_WIN + 1;
+    u32 *ctr = d.a->get<u32>(2);
+    d.zero(ctr, 2 * sizeof(u32));
+    const u64 cap = np / 8 + 1024;  // anchors are ~1/16 of the positions
+    u64 *hk = d.a->get<u64>(cap), *hk2 = d.a->get<u64>(cap);
+    u32 *hv = d.a->get<u32>(cap), *hv2 = d.a->get<u32>(cap);
+    // anchors compacted by a scan (32-bit multiplicative hashing, native
+    // multiplies; no shared counter)
+    const u64 cc = cap;
+    scan_exclusive(
+        d, np, [=] __device__(u64 i) { return (rep_hash8(rank, i) >> 28) == 0 ? 1u : 0u; },
+        [=] __device__(u64 i, u32 slot, u32 v) {
+            if (!v || slot >= cc) return;
+            u32 h1 = 0x85A308D3u, h2 = 0x13198A2Eu;
+            #pragma unroll 4
+            for (int j = 0; j < REP_WIN; j++) {
+                const u32 r = rank[i + j];
+                h1 = (h1 ^ r) * 0x85EBCA6Bu;
+                h2 = (h2 + r) * 0xC2B2AE35u;
+                h2 ^= h2 >> 13;
+            }
+            hk[slot] = ((u64)h1 << 32) | h2;
+            hv[slot] = 0;
+        },
+        ctr);
+    u32 na = d.read_u32(ctr);
+    if (na > cap) na = (u32)cap;  // (a skewed text; the estimate uses the first cap)
+    double share = 0.0;
+    if (na > 1) {
+        bool sw = radix_sort_kv(d, na, hk, hv, hk2, hv2, 64);
+        const u64 *K = sw ? hk2 : hk;
+        const u64 nn = na;
+        u32 *dup = ctr + 1;
+        scan_exclusive(
+            d, na,
+            [=] __device__(u64 i) {
+                return ((i > 0 && K[i - 1] == K[i]) || (i + 1 < nn && K[i + 1] == K[i])) ? 1u : 0u;
+            },
+            [=] __device__(u64, u32, u32) {}, dup);
+        share = (double)d.read_u32(dup) / (double)na;
+    }
+    d.a->reset(mk);
+    return share;
+}
+
+// SA of a level by prefix doubling over all its suffixes: every position is an
+// item, its initial label is the first slot of its symbol's bucket (+1), the
+// sentinel (label 0) is the only terminal, and chain_sort's rounds compare
+// (label(p), label(p+h)) with h doubling.  A suffix that still ties after a
+// round has its first 2h symbols in common with another and no sentinel
+// among them, so p + h stays <= m.  When almost every symbol of the level is
+// distinct (the names of a deep level), few suffixes tie for long and this
+// costs less than naming, recursing and inducing.  Long repeats keep many
+// suffixes tied round after round (prefix doubling is then O(m log m)): when
+// the ties stop shrinking (chain_sort's give-up rule) this returns false,
+// with SA untouched, and the level recurses instead.
+inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *map, const u32 *rank,
+                             const u32 *tails, u32 k) {
+    Dev &d = rt.d;
+    Arena::Mark mk = d.a->mark();
+    u32 *RK = d.a->get<u32>(m + 1);
+    u32 *items = d.a->get<u32>(m);
+    const u64 mm = m;
+    for_each(d, m + 1, [=] __device__(u64 x) {
+        if (x == mm) {
+            RK[x] = 0;
+        } else {
+            const u32 c = rank[x];
+            RK[x] = (c ? tails[c - 1] : 0u) + 1;
+            items[x] = (u32)x;
+        }
+    });
+    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true)) {
+        d.a->reset(mk);
+        return false;
+    }
+    write_sa_from_labels(d, m, RK, SA, map);
+    d.a->reset(mk);
+    return true;
+}
+
 inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
                     const u32 *map, const u32 *rank, const u32 *tails, u32 k) {
     Dev &d = rt.d;
@@ -340,6 +455,19 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         PSK_LAUNCH_CHECK();
         if (map) permute_lms(d, SA, (u32)m, map);
         return;
+    }
+    if (rank && tails && k && rt.doubling_distinct && !rt.doubling_gave_up &&
+        (u64)k * 1024 >= (u64)rt.doubling_distinct * m) {
+        if (repeat_share(d, rank, m) >= 1.0 / 16 ||
+            !level32_doubling(rt, m, SA, map, rank, tails, k)) {
+            // the repeats that kept this level's suffixes tied recur below
+            rt.doubling_gave_up = true;
+        } else {
+            if ((int)rt.levels.size() <= depth) rt.levels.resize(depth + 1);
+            rt.levels[depth] = LevelStat{m, 0, 0, k, 0, 4};
+            rt.levels.resize(depth + 1);
+            return;
+        }
     }
     Arena::Mark mk = d.a->mark();
     const u64 nwords = (m + 31) / 32;

@@ -47,6 +47,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     Dev &d = rt.d;
     rt.levels.clear();
     rt.max_rounds = 0;
+    rt.doubling_gave_up = false;
     if (n == 0) return;
     const u32 *EB = bi ? bi->endbits : nullptr;
     const u32 *SB = bi ? bi->startbits : nullptr;
@@ -287,6 +288,7 @@ inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
     Dev &d = rt.d;
     rt.levels.clear();
     rt.max_rounds = 0;
+    rt.doubling_gave_up = false;
     if (m == 0) return;
     Arena::Mark mk = d.a->mark();
     u32 *sb = d.a->get<u32>((m + 31) / 32 + 1);

@@ -95,6 +95,16 @@ def test_direct_rounds_setting_without_gpu():
             assert e.value.code == P.PSK_EINVAL
 
 
+def test_doubling_setting_without_gpu():
+    with P.Context() as c:
+        for v in (0, 1, 900, 1024):
+            c.set_doubling(v)
+        for v in (-1, 1025):
+            with pytest.raises(P.PskError) as e:
+                c.set_doubling(v)
+            assert e.value.code == P.PSK_EINVAL
+
+
 def test_dict_naming_setting_without_gpu():
     with P.Context() as c:
         c.set_dict_naming(False)

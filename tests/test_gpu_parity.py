@@ -189,6 +189,31 @@ def test_dictionary_naming_cases(case):
     assert np.array_equal(got, oracle.sacak8(t))
 
 
+@pytest.mark.parametrize("threshold", [0, 1, 512, 900, 1024])
+def test_doubling_levels(threshold):
+    # recursion levels sorted by prefix doubling when distinct enough (1 =
+    # every level below level 0, 0 = never): periodic and repetitive texts
+    # keep suffixes tied for many rounds
+    note(f"doubling threshold={threshold}")
+    a, b = b"a", b"ab"
+    while len(b) < 200000:
+        a, b = b, b + a
+    blk = gen.iid_dna(1 << 19, seed=5).tobytes()
+    texts = [gen.markov_dna(1 << 21, seed=3), gen.iid_dna(1 << 20, seed=4), b,
+             b"abcab" * 30000 + b"x" + b"abcab" * 9000, ADVERSARIAL["runs"],
+             gen.uniform8(300000, alphabet=3, seed=6),
+             # every suffix of a copied block ties with its twin: doubling
+             # gives up after a few rounds and the level recurses
+             blk + blk, blk + b"A" + blk + blk[:1000]]
+    with P.Context() as c:
+        c.set_doubling(threshold)
+        for i, t in enumerate(texts):
+            note(f"doubling threshold={threshold} text {i}")
+            assert np.array_equal(P.psacak(t, c), oracle.sacak8(t)), i
+        z = gen.zipf32(1 << 19, sigma=1 << 14, s=1.1, repeat_frac=0.3, seed=8)
+        assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
+
+
 def test_psacak32_random_bucket_pointer_texts(ctx):
     rng = np.random.default_rng(99)
     for it in range(150):
