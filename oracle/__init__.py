@@ -4,10 +4,12 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module, and only as the checker or as the timed
 CPU baseline; the product package ``psacak_amd`` never imports it.
 
-It wraps ``oracle/libpsk_oracle.so`` (built from ``psk_oracle.c`` by
-``oracle/Makefile``), a serial C restatement of hucsmn/psacak 0.1.0:
-``psacak8.rs``, ``psacak32.rs``, ``naming.rs`` and ``common.rs`` (file:line
-citations in the C source).
+It wraps ``oracle/libpsk_oracle.so`` (built from ``psk_oracle.c`` and
+``psk_par.c`` by ``oracle/Makefile``): a serial C restatement of
+hucsmn/psacak 0.1.0 (``psacak8.rs``, ``psacak32.rs``, ``naming.rs``,
+``common.rs``) and a multi-threaded one with the crate's pipelined induce
+(``pipeline.rs``), the timed CPU baseline (file:line citations in the C
+sources).
 """
 from __future__ import annotations
 
@@ -54,6 +56,10 @@ def lib():
             ("or_stage1_8", [_u8p, ctypes.c_uint32, _u32p], ctypes.c_uint32),
             ("or_stage1_32", [_u32p, ctypes.c_uint32, _u32p], ctypes.c_uint32),
             ("or_level_stats8", [_u8p, ctypes.c_uint32, _u64p, ctypes.c_int], ctypes.c_int),
+            ("or_par_sacak8", [_u8p, ctypes.c_uint32, _u32p], ctypes.c_int),
+            ("or_par_sacak32", [_u32p, ctypes.c_uint32, _u32p], ctypes.c_int),
+            ("or_par_threads", [], ctypes.c_int),
+            ("or_par_set_threads", [ctypes.c_int], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -97,6 +103,38 @@ def sacak8(text) -> np.ndarray:
     if rc != 0:
         raise ValueError(f"or_sacak8 failed: {rc}")
     return sa
+
+
+def par_sacak8(text) -> np.ndarray:
+    """psacak8 with the pipelined parallel induce on OpenMP threads
+    (OMP_NUM_THREADS); same SA as :func:`sacak8`."""
+    t = _u8(text)
+    sa = np.empty(len(t), dtype=np.uint32)
+    rc = lib().or_par_sacak8(_p(t, _u8p), len(t), _p(sa, _u32p))
+    if rc != 0:
+        raise ValueError(f"or_par_sacak8 failed: {rc}")
+    return sa
+
+
+def par_sacak32_raw(text) -> np.ndarray:
+    """:func:`sacak32_raw` with the multi-threaded psacak32."""
+    t = _u32(text)
+    tr = translate32(t)
+    if tr is None:
+        return saca_naive32(t)
+    sa = np.empty(len(t), dtype=np.uint32)
+    rc = lib().or_par_sacak32(_p(tr, _u32p), len(tr), _p(sa, _u32p))
+    if rc != 0:
+        raise ValueError(f"or_par_sacak32 failed: {rc}")
+    return sa
+
+
+def par_threads() -> int:
+    return int(lib().or_par_threads())
+
+
+def set_par_threads(p: int) -> None:
+    lib().or_par_set_threads(int(p))
 
 
 def sacak32(text) -> np.ndarray:

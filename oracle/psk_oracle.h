@@ -82,6 +82,16 @@ uint32_t or_stage1_32(const uint32_t *t, uint32_t n, uint32_t *lms_sorted);
  * returns the number of levels (<= max_levels). */
 int or_level_stats8(const uint8_t *t, uint32_t n, uint64_t *stats, int max_levels);
 
+/* Multi-threaded psacak8 (psk_par.c): the levels of or_sacak8 with the
+ * crate's pipelined parallel induce (psacak8.rs:201-344, pipeline.rs:20-358)
+ * on OpenMP threads.  Same SA as or_sacak8.  Timed CPU baseline only. */
+int or_par_sacak8(const uint8_t *t, uint32_t n, uint32_t *sa);
+/* Multi-threaded psacak32 on a bucket-pointer text; same SA as or_sacak32. */
+int or_par_sacak32(const uint32_t *t, uint32_t m, uint32_t *sa);
+/* threads or_par_sacak8 uses (omp_get_max_threads()), and a setter. */
+int or_par_threads(void);
+void or_par_set_threads(int p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -218,3 +218,32 @@ def test_level_stats_shape():
     for a, b in zip(st_, st_[1:]):
         assert b[0] == a[1]  # next level's size = this level's LMS count
     assert st_[-1, 3] == st_[-1, 1]  # the last level names everything uniquely
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_restatement_equals_serial(threads):
+    """or_par_sacak8 (the pipelined parallel induce, timed CPU baseline) gives
+    the serial oracle's SA, below and above the 2-block parallel threshold
+    (psacak8.rs:15), on texts whose induces defer many writes."""
+    rng = np.random.default_rng(11)
+    old = oracle.par_threads()
+    oracle.set_par_threads(threads)
+    try:
+        for n in [0, 1, 2, 3, 4, 9, 1000, (1 << 18) - 1, (1 << 18) + 5, 700_001]:
+            texts = [
+                rng.integers(0, 4, n).astype(np.uint8),
+                rng.integers(0, 256, n).astype(np.uint8),
+                np.full(n, 7, np.uint8),
+                np.tile(np.frombuffer(b"abcab", np.uint8), n // 5 + 1)[:n].copy(),
+            ]
+            if n > 1000:
+                # long repeats drive a deep recursion through the u32 levels
+                base = rng.integers(0, 4, 5000).astype(np.uint8)
+                texts.append(np.tile(base, n // 5000 + 1)[:n].copy())
+            for t in texts:
+                assert np.array_equal(oracle.par_sacak8(t), oracle.sacak8(t)), (n, t[:8])
+        for n in [5, 3000, 400_003]:
+            z = np.minimum(rng.zipf(1.3, n), 1 << 16).astype(np.uint32)
+            assert np.array_equal(oracle.par_sacak32_raw(z), oracle.sacak32_raw(z)), n
+    finally:
+        oracle.set_par_threads(old)
