@@ -106,7 +106,8 @@ int psk_psacak32_raw(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *s
 /* Pointers are device memory on the context's device; stream is a hipStream_t
  * (NULL = the context's own stream).  The call enqueues all work on that stream
  * and returns when the SA is complete on the device (it synchronises the
- * stream); d_sa must hold n u32. */
+ * stream); d_sa must hold n u32 and must not overlap d_text (as the crate's
+ * borrows exclude it). */
 int psk_psacak8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, uint32_t *d_sa,
                     void *stream);
 int psk_psacak32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,

@@ -218,6 +218,10 @@ class Prof {
         (void)hipEventRecord(recs_[rec].b, recs_[rec].s);
     }
     u64 *dyn_slot(int rec) { return rec < 0 ? nullptr : dyn_ + rec; }
+    // bytes of a pending record known on the host only after its launch
+    void add(int rec, double bytes) {
+        if (rec >= 0 && (size_t)rec < recs_.size()) recs_[rec].bytes += bytes;
+    }
 
     // synchronise and fold the pending records into the totals
     void flush() {

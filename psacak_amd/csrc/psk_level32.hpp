@@ -87,13 +87,16 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 // tails (with rank): number of symbols of rank <= c, so the bucket of rank c
 // is [tails[c-1], tails[c]) and the initial labels are its head + 1 (L) or
 // its tail (S, last-slot labels).
-// giveup: return false (RK partly relabelled) when the ties look like long
-// repeats: after round 2 still more than a sixteenth of the items tied and
-// fewer than a quarter of round 1's ties resolved, or after round 4 more than
-// an eighth tied.
+// giveup: return false (RK partly relabelled, SA partly written) when the
+// ties look like long repeats: after round 2 still more than a sixteenth of
+// the items tied and fewer than a quarter of round 1's ties resolved, or after
+// round 4 more than an eighth tied.
+// SA (optional): receives SA[label - 1] = map[p] (or p) for every item as its
+// label becomes final, so the items' SA entries need no pass of their own.
 inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        bool last_slot_labels, const u32 *rank = nullptr,
-                       const u32 *tails = nullptr, u32 k = 0, bool giveup = false) {
+                       const u32 *tails = nullptr, u32 k = 0, bool giveup = false,
+                       u32 *SA = nullptr, const u32 *map = nullptr) {
     Dev &d = rt.d;
     if (A == 0) return true;
     const u32 A0 = A;
@@ -142,6 +145,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
         const u64 AA = A;
+        int pr = -1;
         {
             const bool lastconv = last_slot_labels && first;
             Arena::Mark m3 = d.a->mark();
@@ -159,8 +163,11 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             if (lastconv) k_rmin_tiles<<<1, BLOCK, 0, d.s>>>(te, nt);
             PSK_LAUNCH_CHECK();
             const u32 *gl = (first && !byrank) ? nullptr : glab;
-            int pr = d.pbegin("chain_relabel", 28.0 * AA);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv}, nxt, tg, ts, tk, te,
+            // 28 B per item; the SA entries of the items that finish are
+            // added once their count is known
+            pr = d.pbegin("chain_relabel", 28.0 * AA);
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv, SA, map, mm, err}, nxt,
+                                               tg, ts, tk, te,
                                                tq, lastconv, mm, gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
@@ -168,6 +175,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         }
         u32 ag[2];
         d.read(ag, cnt, sizeof(ag));
+        if (SA) d.padd(pr, (map ? 8.0 : 4.0) * (double)(AA - ag[0]));
         A = ag[0];
         if (nxt != act) std::swap(act, vt);
         std::swap(gid, gid2);
@@ -239,18 +247,6 @@ void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int de
              const u32 *map = nullptr, const u32 *rank = nullptr, const u32 *tails = nullptr,
              u32 k = 0);
 
-// SA of a level from complete labels RK (label = slot + 1); with map, the
-// entries are map[x] instead of x (the caller's positions of this level's
-// symbols: one sequential read instead of a later gather over the SA).
-inline void write_sa_from_labels(Dev &d, u64 m, const u32 *RK, u32 *SA, const u32 *map = nullptr) {
-    u32 *err = d.err;
-    for_each(d, m, [=] __device__(u64 x) {
-        u32 r = RK[x];
-        if (r >= 1 && r <= m) SA[r - 1] = map ? map[x] : (u32)x;
-        else atomicOr(err, 64u);
-    });
-}
-
 // LMS positions in the order of their stage-1 labels (distinct slots
 // 1..m): each LMS position x marks its slot with x + 1 in a zeroed array of
 // m words, then one compaction keeps the marks in slot order.  lms_text is in
@@ -277,9 +273,11 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lms_text
 // The reduced problem of a level from its names (reduced, tails, k): types,
 // bucket-pointer translation and the next level, or the inverse when every
 // name is distinct.  SA1 receives the sorted LMS positions (lms_text maps a
-// reduced index to its position).
+// reduced index to its position), or with map_out false the sorted reduced
+// indices themselves (the caller maps them).
 inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u32 k,
-                            const u32 *lms_text, u32 m1, u32 *SA1, int depth) {
+                            const u32 *lms_text, u32 m1, u32 *SA1, int depth,
+                            bool map_out = true) {
     Dev &d = rt.d;
     Arena::Mark mk = d.a->mark();
     if (k < m1) {
@@ -288,9 +286,9 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
         u32 *s1 = d.a->get<u32>(m1);
         translate_names(d, reduced, s1, m1, sb, tails);
         // the names stay as the dense symbol ranks of the next level
-        level32(rt, s1, m1, sb, SA1, depth + 1, lms_text, reduced, tails, k);
+        level32(rt, s1, m1, sb, SA1, depth + 1, map_out ? lms_text : nullptr, reduced, tails, k);
     } else {
-        inverse_names(d, reduced, m1, SA1, lms_text);
+        inverse_names(d, reduced, m1, SA1, map_out ? lms_text : nullptr);
     }
     d.a->reset(mk);
 }
@@ -299,7 +297,7 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
 template <class C>
 inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
                           const u32 *lmsbits, u64 nwords, u32 *NP, u32 m1, u32 *SA1, int depth,
-                          LevelStat &st, const u32 *EB = nullptr) {
+                          LevelStat &st, const u32 *EB = nullptr, bool map_out = true) {
     Dev &d = rt.d;
     if (m1 == 0) return;
     Arena::Mark mk = d.a->mark();
@@ -311,7 +309,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
                      EB);
     st.k = k;
     rt.levels[depth] = st;
-    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth);
+    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth, map_out);
     d.a->reset(mk);
 }
 
@@ -320,7 +318,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
 template <class RF>
 inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32 *lms_text,
                                  u32 m1, u32 *SA1, int depth, LevelStat &st,
-                                 LmsEnds ends = LmsEnds{}) {
+                                 LmsEnds ends = LmsEnds{}, bool map_out = true) {
     Dev &d = rt.d;
     if (m1 == 0) return true;
     if (rt.direct_rounds == 0) return false;
@@ -341,13 +339,14 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     }
     st.k = k;
     rt.levels[depth] = st;
-    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth);
+    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth, map_out);
     d.a->reset(mk);
     return true;
 }
 
 // One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
-// map (optional): SA entries are written as map[x] (see write_sa_from_labels).
+// map (optional): SA entries are written as map[x] (the caller's positions of
+// this level's symbols) instead of x.
 // rank, tails (optional): dense rank of every symbol of s (k distinct
 // values) and the cumulative symbol counts per rank.
 // Long exact repeats in a level's text (symbols rank[0..m)): the share of
@@ -437,11 +436,10 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *map, const 
             items[x] = (u32)x;
         }
     });
-    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true)) {
+    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true, SA, map)) {
         d.a->reset(mk);
         return false;
     }
-    write_sa_from_labels(d, m, RK, SA, map);
     d.a->reset(mk);
     return true;
 }
@@ -545,10 +543,10 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     labels_L(d, s, m, sbits, RK);
     // sorted LMS suffixes are grouped by first symbol already
     if (m1) labels_lms_at_tails(d, s, SA1, nullptr, m1, RK, cnt);
-    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
+    // every position is an L or an S item: the two sorts write the whole SA
+    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, map);
     labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k);
-    write_sa_from_labels(d, m, RK, SA, map);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, map);
     d.a->reset(mk);
 }
 

@@ -14,20 +14,42 @@ struct BatchInfo {
     const u32 *endbits, *startbits;
 };
 
-// Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87):
-// entry j of SA1 starts with the byte c whose LMS range [lstart[c],
-// lstart[c+1]) holds j (binary search in LDS) and goes to j + lmsoff[c],
-// with the window of the bytes before it (PW) for the induce.
+// An LMS suffix as the final induce places it: its position and the window
+// of the bytes before it (one 16-byte record, fetched by one load).
+struct __align__(16) LmsRec {
+    u64 x, w;
+};
+
+// Records of the LMS positions in text order (lms_text ascending, so the
+// text reads of the windows are sequential).
 template <bool ALIGNED>
+__global__ void __launch_bounds__(BLOCK) k_lms_records(const u32 *lms_text, u32 m1, Pred pr,
+                                                       LmsRec *R) {
+    for (u64 p = (u64)blockIdx.x * BLOCK + threadIdx.x; p < m1; p += (u64)gridDim.x * BLOCK) {
+        const u32 x = lms_text[p];
+        R[p] = LmsRec{x, pr.window<ALIGNED>(x)};
+    }
+}
+
+// Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87):
+// entry j of SA1 (an index into the text-order LMS list) starts with the
+// byte c whose LMS range [lstart[c], lstart[c+1]) holds j (binary search in
+// LDS) and goes to j + lmsoff[c], with its window (PW) for the induce.  One
+// record gather per entry gives both the position and the window.
 __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
-                                                     const u32 *lmsoff, Pred pr, u32 *SA,
-                                                     u64 *PW) {
+                                                     const u32 *lmsoff, const LmsRec *R, u32 *SA,
+                                                     u64 *PW, u32 *err) {
     __shared__ u32 st[257], off[256];
     for (int i = threadIdx.x; i < 257; i += BLOCK) st[i] = lstart[i];
     off[threadIdx.x] = lmsoff[threadIdx.x];
     __syncthreads();
     for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < m1; j += (u64)gridDim.x * BLOCK) {
         u32 p = SA1[j];
+        if (p >= m1) {
+            atomicOr(err, 512u);
+            continue;
+        }
+        const LmsRec r = R[p];
         u32 lo = 0, hi = 256;  // largest c with st[c] <= j; st[256] = m1 > j
         while (hi - lo > 1) {
             u32 mid = (lo + hi) >> 1;
@@ -35,8 +57,8 @@ __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, con
             else hi = mid;
         }
         u32 dst = (u32)j + off[lo];
-        SA[dst] = p;
-        PW[dst] = pr.window<ALIGNED>(p);
+        SA[dst] = (u32)r.x;
+        PW[dst] = r.w;
     }
 }
 
@@ -142,7 +164,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         u8 *crank = d.a->get<u8>(256);
         d.write(crank, hcr, sizeof(hcr));
         named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st,
-                                     ends);
+                                     ends, false);
     }
     if (!named) {
         // induced stage 1 (long repeated LMS substrings, or batch texts whose
@@ -190,7 +212,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             d.a->reset(m2);
         }
         u32 *NP = d.a->get<u32>(n / 2 + 1);
-        solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
+        solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB, false);
     }
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
@@ -206,11 +228,17 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         }
         h[256] = (u32)acc;
         d.write(lstart, h, sizeof(h));
+        // SA1 holds reduced indices (the recursion leaves the mapping to
+        // this level): the records of the LMS positions in text order, then
+        // one record gather per sorted entry
+        LmsRec *R = d.a->get<LmsRec>(m1);
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 65536);
         if ((((uintptr_t)T) & 7u) == 0)
-            k_place_lms<true><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, pr, SA, PW);
+            k_lms_records<true><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
         else
-            k_place_lms<false><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, pr, SA, PW);
+            k_lms_records<false><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
+        PSK_LAUNCH_CHECK();
+        k_place_lms<<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, R, SA, PW, d.err);
         PSK_LAUNCH_CHECK();
     }
     induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);

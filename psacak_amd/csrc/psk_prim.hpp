@@ -25,6 +25,7 @@ struct Dev {
     int pbegin(const char *name, double bytes) { return prof ? prof->begin(s, name, bytes) : -1; }
     void pend(int r) { if (prof) prof->end(r); }
     u64 *pslot(int r) { return prof ? prof->dyn_slot(r) : nullptr; }
+    void padd(int r, double bytes) { if (prof) prof->add(r, bytes); }
 
     u32 read_u32(const u32 *d) {
         PSK_HIP(hipMemcpyAsync(pinned, d, sizeof(u32), hipMemcpyDeviceToHost, s));
