@@ -511,8 +511,10 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     // above, else by their bucket-pointer values (order- and equality-
     // preserving: types inside an LMS substring follow from its symbols)
     const bool named =
-        rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st)
-             : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st);
+        rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st,
+                                    LmsEnds{}, false)
+             : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st,
+                                    LmsEnds{}, false);
     if (!named) {
         // induced stage 1 (long repeated LMS substrings): LMS characters at
         // their bucket tails, in arbitrary order (psacak32.rs:60-99)
@@ -536,13 +538,38 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         u32 *lms_sorted = d.a->get<u32>(m1 + 1);
         extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
         u32 *NP = d.a->get<u32>(m / 2 + 1);
-        solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st);
+        solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st,
+                      nullptr, false);
     }
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
     labels_L(d, s, m, sbits, RK);
     // sorted LMS suffixes are grouped by first symbol already
-    if (m1) labels_lms_at_tails(d, s, SA1, nullptr, m1, RK, cnt);
+    if (m1) {
+        // SA1 holds reduced indices: one 8-byte record (symbol, position) per
+        // LMS position in text order (sequential reads), then one record
+        // gather per sorted entry gives the list and its symbols
+        Arena::Mark m2 = d.a->mark();
+        u64 *R = d.a->get<u64>(m1);
+        u32 *list = d.a->get<u32>(m1);
+        u64 *vals = d.a->get<u64>(m1);
+        for_each(d, m1, [=] __device__(u64 p) {
+            const u32 x = lms_text[p];
+            R[p] = ((u64)s[x] << 32) | x;
+        });
+        u32 *err = d.err;
+        const u32 mm1 = m1;
+        for_each(d, m1, [=] __device__(u64 j) {
+            const u32 p = SA1[j];
+            u64 r = 0;
+            if (p < mm1) r = R[p];
+            else atomicOr(err, 512u);
+            list[j] = (u32)r;
+            vals[j] = r >> 32;
+        });
+        labels_lms_at_tails(d, s, list, vals, m1, RK, cnt);
+        d.a->reset(m2);
+    }
     // every position is an L or an S item: the two sorts write the whole SA
     chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, map);
     labels_S(d, s, m, sbits, RK);
