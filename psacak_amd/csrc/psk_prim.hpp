@@ -184,24 +184,54 @@ __global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
     if (total && threadIdx.x == 0) *total = carry;
 }
 
+// One tile of SCAN_TILE items per workgroup: coalesced loads into LDS, each
+// thread scans SCAN_ROUNDS consecutive items, one block scan of the thread
+// totals, and coalesced calls of out in index order.  (LDS rows are padded by
+// one word per 32 to spread the per-thread runs over the banks.)
 template <class Op, bool INCLUSIVE, class In, class Out>
-__global__ void k_scan_apply(In in, Out out, u64 n, const u32 *tile_off) {
+__global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off) {
+    __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
     __shared__ u32 lds[NWAVES + 1];
-    u64 base = (u64)blockIdx.x * SCAN_TILE;
-    u32 run = tile_off[blockIdx.x];
+    const u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 v[SCAN_ROUNDS];
+    #pragma unroll
     for (int r = 0; r < SCAN_ROUNDS; r++) {
-        u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        u32 v = (i < n) ? in(i) : Op::id();
-        u32 agg;
-        u32 inc = block_incl<Op>(v, lds, &agg);
-        if (i < n) {
-            if (INCLUSIVE) out(i, Op::op(run, inc), v);
-            else {
-                // exclusive prefix for a sum scan: inclusive minus own value
-                out(i, run + inc - v, v);
-            }
-        }
-        run = Op::op(run, agg);
+        const u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        v[r] = (i < n) ? in(i) : Op::id();
+        const u32 j = (u32)r * BLOCK + threadIdx.x;
+        buf[j + (j >> 5)] = v[r];
+    }
+    __syncthreads();
+    const u32 j0 = threadIdx.x * SCAN_ROUNDS;
+    u32 loc = Op::id();
+    #pragma unroll
+    for (int k = 0; k < SCAN_ROUNDS; k++) {
+        const u32 j = j0 + k;
+        loc = Op::op(loc, buf[j + (j >> 5)]);
+    }
+    u32 agg;
+    const u32 inc = block_incl<Op>(loc, lds, &agg);
+    // exclusive prefix of this thread = inclusive of the previous thread
+    u32 ex = __shfl_up(inc, 1, WAVE);
+    if (lane_id() == 0) {
+        ex = Op::id();
+        for (int w = 0; w < wave_id(); w++) ex = Op::op(ex, lds[w]);
+    }
+    u32 run = Op::op(tile_off[blockIdx.x], ex);
+    #pragma unroll
+    for (int k = 0; k < SCAN_ROUNDS; k++) {
+        const u32 j = j0 + k, idx = j + (j >> 5);
+        const u32 x = buf[idx];
+        run = Op::op(run, x);
+        // exclusive prefix for a sum scan: inclusive minus own value
+        buf[idx] = INCLUSIVE ? run : run - x;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        const u64 i = base + (u64)r * BLOCK + threadIdx.x;
+        const u32 j = (u32)r * BLOCK + threadIdx.x;
+        if (i < n) out(i, buf[j + (j >> 5)], v[r]);
     }
 }
 
