@@ -56,6 +56,10 @@ void psk_ctx_destroy(psk_ctx *ctx);
 const char *psk_last_error(const psk_ctx *ctx);
 /* Release cached device workspace (keeps the stream). */
 int psk_ctx_trim(psk_ctx *ctx);
+/* Device workspace the context holds now and the most it has held (bytes);
+ * either pointer may be NULL.  No counterpart in the crate, which uses the
+ * caller's SA plus O(block) buffers: for sizing HBM budgets. */
+int psk_ctx_workspace(const psk_ctx *ctx, uint64_t *reserved, uint64_t *peak);
 /* Naming strategy of this implementation (no counterpart in the crate, whose
  * naming always follows induced sorting, psacak8.rs:57-72, naming.rs:20-259):
  * LMS substrings are named by sorting them directly for at most `rounds`

@@ -68,6 +68,7 @@ def _load():
         "psk_ctx_destroy": ([vp], None),
         "psk_last_error": ([vp], ctypes.c_char_p),
         "psk_ctx_trim": ([vp], i32),
+        "psk_ctx_workspace": ([vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], i32),
         "psk_ctx_set_direct_rounds": ([vp, i32], i32),
         "psk_ctx_set_dict_naming": ([vp, i32], i32),
         "psk_ctx_set_doubling": ([vp, i32], i32),
@@ -178,6 +179,12 @@ class Context:
 
     def trim(self):
         self._check(_lib.psk_ctx_trim(self._h))
+
+    def workspace(self):
+        """(reserved, peak) device workspace of this context, in bytes."""
+        r, p = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(_lib.psk_ctx_workspace(self._h, ctypes.byref(r), ctypes.byref(p)))
+        return int(r.value), int(p.value)
 
     def set_direct_rounds(self, rounds: int):
         """Rounds of direct LMS-substring sorting before induced sorting takes

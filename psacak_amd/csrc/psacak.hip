@@ -242,11 +242,20 @@ int psk_ctx_trim(psk_ctx *ctx) {
     });
 }
 
-// Admission footprint: level 0 holds type/LMS bit vectors (n/4), the digit
-// scratch (n), LMS lists and rank table (<= 4 x 2n), plus the recursion on
-// <= n/2 symbols; each u32 level needs <= ~36 bytes per symbol.  Bounded by
-// the worst case m1 = n/2 at every level.
-uint64_t psk_workspace_bytes32(uint64_t n) {
+int psk_ctx_workspace(const psk_ctx *ctx, uint64_t *reserved, uint64_t *peak) {
+    if (!ctx) return PSK_EINVAL;
+    if (reserved) *reserved = ctx->arena.reserved();
+    if (peak) *peak = ctx->arena.peak();
+    return PSK_OK;
+}
+
+// Workspace bounds: level 0 holds type/LMS bit vectors (n/4), the digit
+// scratch (n), LMS lists and rank table (<= 4 x 2n), the placement records
+// (16 bytes per LMS suffix, <= 8n) unless the context is lean, plus the
+// recursion on <= n/2 symbols; each u32 level needs <= ~36 bytes per symbol.
+// Bounded by the worst case m1 = n/2 at every level (measured peaks: 31-37
+// bytes per input byte lean, 36-42 with records).
+static uint64_t levels_bytes(uint64_t n) {
     uint64_t total = 0, m = n;
     while (m > 3) {
         total += m * 36 + (64ull << 20);
@@ -255,10 +264,19 @@ uint64_t psk_workspace_bytes32(uint64_t n) {
     return total + (64ull << 20);
 }
 
-uint64_t psk_workspace_bytes8(uint64_t n) {
-    // bit vectors, digit cache, byte windows (8n), LMS lists and names
-    return n / 4 + n + 17 * n + psk_workspace_bytes32(n / 2) + (64ull << 20);
+uint64_t psk_workspace_bytes32(uint64_t n) {
+    // the translation of an arbitrary text into bucket-pointer form at the
+    // first level adds about 4 bytes per symbol (measured: 81.7 bytes per
+    // symbol for psacak32_raw on 2^28 Zipf symbols)
+    return levels_bytes(n) + 4 * n;
 }
+
+static uint64_t lean_bytes8(uint64_t n) {
+    // bit vectors, digit cache, byte windows (8n), LMS lists and names
+    return n / 4 + n + 17 * n + levels_bytes(n / 2) + (64ull << 20);
+}
+
+uint64_t psk_workspace_bytes8(uint64_t n) { return lean_bytes8(n) + 8 * n; }
 
 int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa,
                        uint64_t sa_len) {
@@ -534,15 +552,20 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
 // ---- serving ----------------------------------------------------------------
 
 uint64_t psk_server_footprint(uint64_t bytes, int batched) {
-    // text + SA + level workspace, plus the text-id partition of a batch
-    return psk_workspace_bytes8(bytes) + 5 * bytes + (batched ? 24 * bytes : 0);
+    // text + SA + level workspace (the workers' contexts are lean), plus the
+    // text-id partition of a batch
+    return lean_bytes8(bytes) + 5 * bytes + (batched ? 24 * bytes : 0);
 }
 
+// Worker loop.  psk_ctx_trim frees device memory, and hipFree waits for the
+// whole device: it is always called with the server lock released, so the
+// other workers keep admitting, finishing and launching jobs meanwhile.
 static void server_worker(psk_server *s, int w) {
     psk_ctx *c = s->ctxs[w];
     for (;;) {
         std::vector<SrvReq *> job;
         u64 foot = 0;
+        bool trim = false;  // give back this worker's cached chunks first
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -573,8 +596,9 @@ static void server_worker(psk_server *s, int w) {
                     const u64 mine = c->arena.reserved();
                     bool admit = s->budget == 0 || others + std::max(mine, f) <= s->budget;
                     if (!admit && mine && others + f <= s->budget) {
-                        // this worker's cached chunks are what stands in the way
-                        (void)psk_ctx_trim(c);
+                        // this worker's cached chunks are what stands in the
+                        // way: take the job, trim before running it
+                        trim = true;
                         admit = true;
                     }
                     if (admit) {
@@ -586,15 +610,22 @@ static void server_worker(psk_server *s, int w) {
                         break;
                     }
                     if (mine) {
-                        // idle and blocked: give back the cache, let the others retry
-                        (void)psk_ctx_trim(c);
-                        s->cv_work.notify_all();
-                        continue;
+                        // idle and blocked: give back the cache (below, with
+                        // the lock released), then look again
+                        trim = true;
+                        break;
                     }
                     // idle workers' caches can be given back: wake them
                     if (idle_cache) s->cv_work.notify_all();
                 }
                 s->cv_work.wait(lk);
+            }
+        }
+        if (trim) {
+            (void)psk_ctx_trim(c);
+            if (job.empty()) {
+                s->cv_work.notify_all();
+                continue;
             }
         }
         int rc;
@@ -613,6 +644,7 @@ static void server_worker(psk_server *s, int w) {
             rc = psk_batch8(c, tp.data(), lp.data(), sp.data(), job.size());
         }
         double t = now_ms();
+        bool trim_after = false;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (SrvReq *r : job) {
@@ -628,10 +660,10 @@ static void server_worker(psk_server *s, int w) {
             s->peak = std::max(s->peak, res);
             // cached chunks of finished jobs are not in flight: a worker keeps
             // at most its share of the budget between jobs
-            if (s->budget && (res > s->budget || c->arena.reserved() > s->budget / s->nworkers))
-                (void)psk_ctx_trim(c);
+            trim_after = s->budget && (res > s->budget || c->arena.reserved() > s->budget / s->nworkers);
         }
         s->cv_done.notify_all();
+        if (trim_after) (void)psk_ctx_trim(c);
         s->cv_work.notify_all();
     }
 }
@@ -650,6 +682,11 @@ int psk_server_create(psk_server **out, uint64_t hbm_budget_bytes, int max_worke
             delete s;
             return rc;
         }
+        // lean contexts: no placement records.  Workers take jobs of every
+        // size one after another, and each size lays its chunks out
+        // differently, so every byte a job does not need is a byte of
+        // fragmentation across jobs (and trims, which wait for the device).
+        c->rt.lms_records = false;
         s->ctxs.push_back(c);
     }
     s->wfoot.assign(max_workers, 0);

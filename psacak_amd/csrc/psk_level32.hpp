@@ -48,6 +48,11 @@ struct Runtime {
     // round 1 of direct naming at level 0 through a dictionary of the
     // distinct keys (false: by sorting every key)
     bool dict_naming = true;
+    // level 0 places its sorted LMS suffixes from 16-byte (position, window)
+    // records gathered by reduced index (one random read each, 16 bytes per
+    // LMS suffix of extra workspace), or, when false, by gathering the
+    // position and then the window (two random reads, no extra workspace)
+    bool lms_records = true;
 };
 
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.
@@ -550,9 +555,10 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         // LMS position in text order (sequential reads), then one record
         // gather per sorted entry gives the list and its symbols
         Arena::Mark m2 = d.a->mark();
-        u64 *R = d.a->get<u64>(m1);
-        u32 *list = d.a->get<u32>(m1);
-        u64 *vals = d.a->get<u64>(m1);
+        // one block: R, vals, list
+        u64 *R = d.a->get<u64>(2ull * m1 + m1 / 2 + 1);
+        u64 *vals = R + m1;
+        u32 *list = (u32 *)(vals + m1);
         for_each(d, m1, [=] __device__(u64 p) {
             const u32 x = lms_text[p];
             R[p] = ((u64)s[x] << 32) | x;

@@ -31,34 +31,65 @@ __global__ void __launch_bounds__(BLOCK) k_lms_records(const u32 *lms_text, u32 
     }
 }
 
-// Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87):
-// entry j of SA1 (an index into the text-order LMS list) starts with the
-// byte c whose LMS range [lstart[c], lstart[c+1]) holds j (binary search in
-// LDS) and goes to j + lmsoff[c], with its window (PW) for the induce.  One
-// record gather per entry gives both the position and the window.
-__global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
-                                                     const u32 *lmsoff, const LmsRec *R, u32 *SA,
-                                                     u64 *PW, u32 *err) {
+// The bucket of sorted LMS entry j: the largest c with st[c] <= j (st[256] =
+// m1 > j), by binary search in LDS.
+__device__ __forceinline__ u32 lms_bucket(const u32 *st, u32 j) {
+    u32 lo = 0, hi = 256;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (st[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Sorted LMS suffixes to their bucket tails (put_lmssuffixes psacak8.rs:76-87)
+// from records: entry j of SA1 (an index into the text-order LMS list) starts
+// with the byte c whose LMS range [lstart[c], lstart[c+1]) holds j and goes to
+// j + lmsoff[c]; one record gather gives its position and window.
+__global__ void __launch_bounds__(BLOCK) k_place_lms_rec(const u32 *SA1, u32 m1, const u32 *lstart,
+                                                         const u32 *lmsoff, const LmsRec *R,
+                                                         u32 *SA, u64 *PW, u32 *err) {
     __shared__ u32 st[257], off[256];
     for (int i = threadIdx.x; i < 257; i += BLOCK) st[i] = lstart[i];
     off[threadIdx.x] = lmsoff[threadIdx.x];
     __syncthreads();
     for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < m1; j += (u64)gridDim.x * BLOCK) {
-        u32 p = SA1[j];
-        if (p >= m1) {
+        const u32 q = SA1[j];
+        if (q >= m1) {
             atomicOr(err, 512u);
             continue;
         }
-        const LmsRec r = R[p];
-        u32 lo = 0, hi = 256;  // largest c with st[c] <= j; st[256] = m1 > j
-        while (hi - lo > 1) {
-            u32 mid = (lo + hi) >> 1;
-            if (st[mid] <= (u32)j) lo = mid;
-            else hi = mid;
-        }
-        u32 dst = (u32)j + off[lo];
+        const LmsRec r = R[q];
+        const u32 dst = (u32)j + off[lms_bucket(st, (u32)j)];
         SA[dst] = (u32)r.x;
         PW[dst] = r.w;
+    }
+}
+
+// The same placement without records: entry j of SA1 (an index into the
+// text-order LMS list) starts with the
+// byte c whose LMS range [lstart[c], lstart[c+1]) holds j (binary search in
+// LDS) and goes to j + lmsoff[c] as lms_text[SA1[j]], with the window of the
+// bytes before it (PW) for the induce.
+template <bool ALIGNED>
+__global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
+                                                     const u32 *lmsoff, const u32 *lms_text,
+                                                     Pred pr, u32 *SA, u64 *PW, u32 *err) {
+    __shared__ u32 st[257], off[256];
+    for (int i = threadIdx.x; i < 257; i += BLOCK) st[i] = lstart[i];
+    off[threadIdx.x] = lmsoff[threadIdx.x];
+    __syncthreads();
+    for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < m1; j += (u64)gridDim.x * BLOCK) {
+        const u32 q = SA1[j];
+        if (q >= m1) {
+            atomicOr(err, 512u);
+            continue;
+        }
+        const u32 p = lms_text[q];
+        const u32 dst = (u32)j + off[lms_bucket(st, (u32)j)];
+        SA[dst] = p;
+        PW[dst] = pr.window<ALIGNED>(p);
     }
 }
 
@@ -142,6 +173,10 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
 
     // ---- LMS-substring names and the reduced problem (psacak8.rs:33-48) ------
     u32 *SA1 = d.a->get<u32>((u64)m1 + 1);
+    // placement records (filled after the recursion), held from here:
+    // allocated after the recursion they would need a chunk of their own that
+    // no other buffer uses, call after call
+    LmsRec *R = rt.lms_records ? d.a->get<LmsRec>(m1) : nullptr;
     bool named = false;
     {
         // by sorting the substrings, bytes ranked among those present; in a
@@ -229,16 +264,22 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         h[256] = (u32)acc;
         d.write(lstart, h, sizeof(h));
         // SA1 holds reduced indices (the recursion leaves the mapping to
-        // this level): the records of the LMS positions in text order, then
-        // one record gather per sorted entry
-        LmsRec *R = d.a->get<LmsRec>(m1);
+        // this level)
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 65536);
-        if ((((uintptr_t)T) & 7u) == 0)
-            k_lms_records<true><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
-        else
-            k_lms_records<false><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
-        PSK_LAUNCH_CHECK();
-        k_place_lms<<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, R, SA, PW, d.err);
+        const bool al = (((uintptr_t)T) & 7u) == 0;
+        if (R) {
+            if (al) k_lms_records<true><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
+            else k_lms_records<false><<<(u32)blocks, BLOCK, 0, d.s>>>(lms_text, m1, pr, R);
+            PSK_LAUNCH_CHECK();
+            k_place_lms_rec<<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, R, SA, PW,
+                                                             d.err);
+        } else if (al) {
+            k_place_lms<true><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, lms_text, pr,
+                                                              SA, PW, d.err);
+        } else {
+            k_place_lms<false><<<(u32)blocks, BLOCK, 0, d.s>>>(SA1, m1, lstart, lmsoff, lms_text,
+                                                               pr, SA, PW, d.err);
+        }
         PSK_LAUNCH_CHECK();
     }
     induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);

@@ -414,6 +414,44 @@ def test_cli_matches_crate_output_format(tmp_path, capsys):
     assert err[3] == "check: true"
 
 
+@pytest.mark.parametrize("kind,n", [("iid", 64 << 20), ("markov", 256 << 20)])
+def test_workspace_within_estimate(kind, n):
+    """The workspace a call reserves stays within psk_workspace_bytes8, which
+    the server's HBM admission relies on; a second call of another size on
+    the same context reuses the chunks (no growth past the estimate)."""
+    note(f"workspace {kind} {n}")
+    import torch
+    dev = torch.device("cuda:0")
+    t = gen.iid_dna(n, seed=5) if kind == "iid" else gen.markov_dna(n, seed=5)
+    with P.Context() as c:
+        for m in (n, n // 2):
+            d_t = torch.from_numpy(t[:m].copy()).to(dev)
+            d_sa = torch.empty(m, dtype=torch.int32, device=dev)
+            P.psacak8_dev(d_t, d_sa, ctx=c)
+            torch.cuda.synchronize()
+            assert P.check8_dev(d_t, d_sa)  # the default context, not c
+        reserved, peak = c.workspace()
+        assert 0 < reserved <= peak <= P.workspace_bytes8(n), (reserved, peak)
+
+
+def test_workspace32_within_estimate():
+    """psacak32_raw (translation included) stays within psk_workspace_bytes32."""
+    note("workspace32")
+    import torch
+    dev = torch.device("cuda:0")
+    n = 1 << 24
+    z = gen.zipf32(n, seed=3)
+    with P.Context() as c:
+        d_t = torch.from_numpy(z.view(np.int32)).to(dev)
+        d_sa = torch.empty(n, dtype=torch.int32, device=dev)
+        P.psacak32_raw_dev(d_t, d_sa, ctx=c)
+        torch.cuda.synchronize()
+        sa = d_sa.cpu().numpy().view(np.uint32)
+        assert np.array_equal(np.sort(sa), np.arange(n, dtype=np.uint32))
+        reserved, peak = c.workspace()
+        assert 0 < reserved <= peak <= P.workspace_bytes32(n), (reserved, peak)
+
+
 def test_budget_is_enforced():
     note("budget")
     with P.Context(hbm_budget=1 << 20) as c:
