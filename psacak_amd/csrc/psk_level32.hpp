@@ -96,12 +96,12 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 // ties look like long repeats: after round 2 still more than a sixteenth of
 // the items tied and fewer than a quarter of round 1's ties resolved, or after
 // round 4 more than an eighth tied.
-// SA (optional): receives SA[label - 1] = map[p] (or p) for every item as its
-// label becomes final, so the items' SA entries need no pass of their own.
+// SA (optional): receives SA[label - 1] = p for every item as its label
+// becomes final, so the items' SA entries need no pass of their own.
 inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        bool last_slot_labels, const u32 *rank = nullptr,
                        const u32 *tails = nullptr, u32 k = 0, bool giveup = false,
-                       u32 *SA = nullptr, const u32 *map = nullptr) {
+                       u32 *SA = nullptr) {
     Dev &d = rt.d;
     if (A == 0) return true;
     const u32 A0 = A;
@@ -171,7 +171,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             // 28 B per item; the SA entries of the items that finish are
             // added once their count is known
             pr = d.pbegin("chain_relabel", 28.0 * AA);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv, SA, map, mm, err}, nxt,
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv, SA, mm, err}, nxt,
                                                tg, ts, tk, te,
                                                tq, lastconv, mm, gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
@@ -180,7 +180,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         }
         u32 ag[2];
         d.read(ag, cnt, sizeof(ag));
-        if (SA) d.padd(pr, (map ? 8.0 : 4.0) * (double)(AA - ag[0]));
+        if (SA) d.padd(pr, 4.0 * (double)(AA - ag[0]));
         A = ag[0];
         if (nxt != act) std::swap(act, vt);
         std::swap(gid, gid2);
@@ -249,8 +249,7 @@ inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64
 }
 
 void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
-             const u32 *map = nullptr, const u32 *rank = nullptr, const u32 *tails = nullptr,
-             u32 k = 0);
+             const u32 *rank = nullptr, const u32 *tails = nullptr, u32 k = 0);
 
 // LMS positions in the order of their stage-1 labels (distinct slots
 // 1..m): each LMS position x marks its slot with x + 1 in a zeroed array of
@@ -277,12 +276,11 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lms_text
 
 // The reduced problem of a level from its names (reduced, tails, k): types,
 // bucket-pointer translation and the next level, or the inverse when every
-// name is distinct.  SA1 receives the sorted LMS positions (lms_text maps a
-// reduced index to its position), or with map_out false the sorted reduced
-// indices themselves (the caller maps them).
-inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u32 k,
-                            const u32 *lms_text, u32 m1, u32 *SA1, int depth,
-                            bool map_out = true) {
+// name is distinct.  SA1 receives the sorted reduced indices (indices into
+// the level's text-order LMS list): the level maps them to positions itself,
+// through records that cost one random read per entry.
+inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u32 k, u32 m1,
+                            u32 *SA1, int depth) {
     Dev &d = rt.d;
     Arena::Mark mk = d.a->mark();
     if (k < m1) {
@@ -291,9 +289,9 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
         u32 *s1 = d.a->get<u32>(m1);
         translate_names(d, reduced, s1, m1, sb, tails);
         // the names stay as the dense symbol ranks of the next level
-        level32(rt, s1, m1, sb, SA1, depth + 1, map_out ? lms_text : nullptr, reduced, tails, k);
+        level32(rt, s1, m1, sb, SA1, depth + 1, reduced, tails, k);
     } else {
-        inverse_names(d, reduced, m1, SA1, map_out ? lms_text : nullptr);
+        inverse_names(d, reduced, m1, SA1);
     }
     d.a->reset(mk);
 }
@@ -302,7 +300,7 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
 template <class C>
 inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const u32 *lms_text,
                           const u32 *lmsbits, u64 nwords, u32 *NP, u32 m1, u32 *SA1, int depth,
-                          LevelStat &st, const u32 *EB = nullptr, bool map_out = true) {
+                          LevelStat &st, const u32 *EB = nullptr) {
     Dev &d = rt.d;
     if (m1 == 0) return;
     Arena::Mark mk = d.a->mark();
@@ -314,7 +312,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
                      EB);
     st.k = k;
     rt.levels[depth] = st;
-    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth, map_out);
+    recurse_reduced(rt, reduced, tails, k, m1, SA1, depth);
     d.a->reset(mk);
 }
 
@@ -323,7 +321,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
 template <class RF>
 inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32 *lms_text,
                                  u32 m1, u32 *SA1, int depth, LevelStat &st,
-                                 LmsEnds ends = LmsEnds{}, bool map_out = true) {
+                                 LmsEnds ends = LmsEnds{}) {
     Dev &d = rt.d;
     if (m1 == 0) return true;
     if (rt.direct_rounds == 0) return false;
@@ -344,16 +342,15 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     }
     st.k = k;
     rt.levels[depth] = st;
-    recurse_reduced(rt, reduced, tails, k, lms_text, m1, SA1, depth, map_out);
+    recurse_reduced(rt, reduced, tails, k, m1, SA1, depth);
     d.a->reset(mk);
     return true;
 }
 
-// One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits.
-// map (optional): SA entries are written as map[x] (the caller's positions of
-// this level's symbols) instead of x.
-// rank, tails (optional): dense rank of every symbol of s (k distinct
-// values) and the cumulative symbol counts per rank.
+// One psacak32 level: s[0..m) in bucket-pointer form with its S-type bits;
+// SA receives the level's SA (positions of s).  rank, tails (optional):
+// dense rank of every symbol of s (k distinct values) and the cumulative
+// symbol counts per rank.
 // Long exact repeats in a level's text (symbols rank[0..m)): the share of
 // 32-symbol windows that occur more than once, estimated on content-defined
 // anchors (positions whose 8-symbol hash is 0 mod 16, so the copies of a
@@ -425,8 +422,8 @@ inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
 // suffixes tied round after round (prefix doubling is then O(m log m)): when
 // the ties stop shrinking (chain_sort's give-up rule) this returns false,
 // with SA untouched, and the level recurses instead.
-inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *map, const u32 *rank,
-                             const u32 *tails, u32 k) {
+inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const u32 *tails,
+                             u32 k) {
     Dev &d = rt.d;
     Arena::Mark mk = d.a->mark();
     u32 *RK = d.a->get<u32>(m + 1);
@@ -441,7 +438,7 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *map, const 
             items[x] = (u32)x;
         }
     });
-    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true, SA, map)) {
+    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true, SA)) {
         d.a->reset(mk);
         return false;
     }
@@ -450,19 +447,18 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *map, const 
 }
 
 inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
-                    const u32 *map, const u32 *rank, const u32 *tails, u32 k) {
+                    const u32 *rank, const u32 *tails, u32 k) {
     Dev &d = rt.d;
     if (m == 0) return;
     if (m <= 3) {
         k_naive_small<u32><<<1, 1, 0, d.s>>>(s, (u32)m, SA);
         PSK_LAUNCH_CHECK();
-        if (map) permute_lms(d, SA, (u32)m, map);
         return;
     }
     if (rank && tails && k && rt.doubling_distinct && !rt.doubling_gave_up &&
         (u64)k * 1024 >= (u64)rt.doubling_distinct * m) {
         if (repeat_share(d, rank, m) >= 1.0 / 16 ||
-            !level32_doubling(rt, m, SA, map, rank, tails, k)) {
+            !level32_doubling(rt, m, SA, rank, tails, k)) {
             // the repeats that kept this level's suffixes tied recur below
             rt.doubling_gave_up = true;
         } else {
@@ -516,10 +512,8 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     // above, else by their bucket-pointer values (order- and equality-
     // preserving: types inside an LMS substring follow from its symbols)
     const bool named =
-        rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st,
-                                    LmsEnds{}, false)
-             : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st,
-                                    LmsEnds{}, false);
+        rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st)
+             : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st);
     if (!named) {
         // induced stage 1 (long repeated LMS substrings): LMS characters at
         // their bucket tails, in arbitrary order (psacak32.rs:60-99)
@@ -543,8 +537,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         u32 *lms_sorted = d.a->get<u32>(m1 + 1);
         extract_lms_sorted(d, m, RK, lms_text, m1, SA, lms_sorted);  // SA is scratch here
         u32 *NP = d.a->get<u32>(m / 2 + 1);
-        solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st,
-                      nullptr, false);
+        solve_reduced(rt, s, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, depth, st);
     }
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
@@ -577,9 +570,9 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         d.a->reset(m2);
     }
     // every position is an L or an S item: the two sorts write the whole SA
-    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, map);
+    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA);
     labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, map);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA);
     d.a->reset(mk);
 }
 

@@ -199,7 +199,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         u8 *crank = d.a->get<u8>(256);
         d.write(crank, hcr, sizeof(hcr));
         named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st,
-                                     ends, false);
+                                     ends);
     }
     if (!named) {
         // induced stage 1 (long repeated LMS substrings, or batch texts whose
@@ -247,7 +247,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             d.a->reset(m2);
         }
         u32 *NP = d.a->get<u32>(n / 2 + 1);
-        solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB, false);
+        solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
     }
 
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)

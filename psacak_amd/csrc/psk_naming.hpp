@@ -150,15 +150,10 @@ inline void translate_names(Dev &d, const u32 *names, u32 *out, u32 m1, const u3
     });
 }
 
-// k == m1: every name is unique, so the reduced text is its own inverse SA;
-// entries are written as map[i] (LMS positions) when map is given.
-inline void inverse_names(Dev &d, const u32 *reduced, u32 m1, u32 *SA1, const u32 *map = nullptr) {
-    for_each(d, m1, [=] __device__(u64 i) { SA1[reduced[i]] = map ? map[i] : (u32)i; });
-}
-
-// SA1 (indices into the text-order LMS list) -> LMS positions, in place.
-inline void permute_lms(Dev &d, u32 *SA1, u32 m1, const u32 *lms_text) {
-    for_each(d, m1, [=] __device__(u64 j) { SA1[j] = lms_text[SA1[j]]; });
+// k == m1: every name is unique, so the reduced text is its own inverse SA
+// (psacak8.rs:41-47): SA1[reduced[i]] = i.
+inline void inverse_names(Dev &d, const u32 *reduced, u32 m1, u32 *SA1) {
+    for_each(d, m1, [=] __device__(u64 i) { SA1[reduced[i]] = (u32)i; });
 }
 
 // Brute-force SA for n <= 3 (saca_tiny, common.rs:13-25).

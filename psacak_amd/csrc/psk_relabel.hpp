@@ -38,21 +38,20 @@ struct RelIn {
 // What a round does with an item's new label.  ChainSink: positions' labels
 // (the first subgroup of a group keeps its label, so it is not rewritten);
 // with SA, an item that is not kept (alone in its subgroup: its label is its
-// final slot + 1) also writes its SA entry, map[p] or p.  Items leave the
-// round in sorted order, so those writes land close together.
+// final slot + 1) also writes its SA entry p.  Items leave the round in
+// sorted order, so those writes land close together.
 // FinishSink: an item that is not kept records its subgroup's slot, and the
 // subgroup's first item marks that slot as a group start.
 struct ChainSink {
     u32 *RK;
     bool lastconv;
     u32 *SA;
-    const u32 *map;
     u64 m;
     u32 *err;
     __device__ __forceinline__ void operator()(u32 p, u32 nl, bool first_sub, bool kept, bool) const {
         if (lastconv || !first_sub) RK[p] = nl;
         if (SA && !kept) {
-            if (nl >= 1 && nl <= m) SA[nl - 1] = map ? map[p] : p;
+            if (nl >= 1 && nl <= m) SA[nl - 1] = p;
             else atomicOr(err, 64u);
         }
     }
