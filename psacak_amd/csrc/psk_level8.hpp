@@ -376,12 +376,23 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
     u64 *k = d.a->get<u64>(n), *kt = d.a->get<u64>(n);
     u32 *v = d.a->get<u32>(n), *vt = d.a->get<u32>(n);
     u32 *first = d.a->get<u32>(n);
-    u32 *ctr = d.a->get<u32>(1);
-    for_each(d, n, [=] __device__(u64 i) {
-        k[i] = t[i];
-        v[i] = (u32)i;
+    u32 *ctr = d.a->get<u32>(2);
+    u32 *mx = ctr + 1;
+    d.zero(mx, sizeof(u32));
+    // keys, and the largest symbol (the sort needs only its bits); a wave
+    // raises the maximum only when it beats the value already stored, so the
+    // same-address atomics stay few
+    for_each_waves(d, n, [=] __device__(u64 i, bool live) {
+        const u32 x = live ? t[i] : 0u;
+        if (live) {
+            k[i] = x;
+            v[i] = (u32)i;
+        }
+        const u32 wm = wave_max(x);
+        if (lane_id() == 0 && wm > __atomic_load_n(mx, __ATOMIC_RELAXED)) atomicMax(mx, wm);
     });
-    bool sw = radix_sort_kv(d, n, k, v, kt, vt, 32);
+    const int bits = std::max(1, bit_width(d.read_u32(mx)));
+    bool sw = radix_sort_kv(d, n, k, v, kt, vt, bits);
     const u64 *K = sw ? kt : k;
     const u32 *V = sw ? vt : v;
     const u64 nn = n;
