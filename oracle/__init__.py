@@ -29,8 +29,9 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 
 
 def build() -> str:
-    """Compile the oracle library with the committed Makefile."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    """Compile the oracle library with the committed Makefile (always, not
+    only when a source is newer than the library)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
 
 
