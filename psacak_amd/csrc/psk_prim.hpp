@@ -188,11 +188,24 @@ __global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
 // thread scans SCAN_ROUNDS consecutive items, one block scan of the thread
 // totals, and coalesced calls of out in index order.  (LDS rows are padded by
 // one word per 32 to spread the per-thread runs over the banks.)
-template <class Op, bool INCLUSIVE, class In, class Out>
-__global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off) {
+// FUSED: tile_off holds the tiles' aggregates, not their scan; each
+// workgroup combines those of the tiles before it (few tiles: a read of at
+// most SCAN_FUSE_MAX words from L2 instead of a single-workgroup launch), and
+// the last one writes the grand aggregate to *total (if non-null).
+template <class Op, bool INCLUSIVE, bool FUSED, class In, class Out>
+__global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off,
+                                                      u32 *total) {
     __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
     __shared__ u32 lds[NWAVES + 1];
     const u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 tbase;
+    if (FUSED) {
+        u32 acc = Op::id();
+        for (u32 i = threadIdx.x; i < blockIdx.x; i += BLOCK) acc = Op::op(acc, tile_off[i]);
+        block_incl<Op>(acc, lds, &tbase);  // includes barriers
+    } else {
+        tbase = tile_off[blockIdx.x];
+    }
     u32 v[SCAN_ROUNDS];
     #pragma unroll
     for (int r = 0; r < SCAN_ROUNDS; r++) {
@@ -217,7 +230,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, con
         ex = Op::id();
         for (int w = 0; w < wave_id(); w++) ex = Op::op(ex, lds[w]);
     }
-    u32 run = Op::op(tile_off[blockIdx.x], ex);
+    u32 run = Op::op(tbase, ex);
     #pragma unroll
     for (int k = 0; k < SCAN_ROUNDS; k++) {
         const u32 j = j0 + k, idx = j + (j >> 5);
@@ -233,7 +246,12 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, con
         const u32 j = (u32)r * BLOCK + threadIdx.x;
         if (i < n) out(i, buf[j + (j >> 5)], v[r]);
     }
+    if (FUSED && total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = Op::op(tbase, agg);
 }
+
+// scans of at most this many tiles combine the tile aggregates inside the
+// apply kernel (two launches instead of three)
+constexpr u32 SCAN_FUSE_MAX = 4096;
 
 // Exclusive sum scan; d_total (device, may be null) receives the sum.
 template <class In, class Out>
@@ -247,9 +265,13 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
     u32 *ta = d.a->get<u32>(nt);
     k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
     PSK_LAUNCH_CHECK();
-    k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, d_total);
-    PSK_LAUNCH_CHECK();
-    k_scan_apply<OpSum, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta);
+    if (nt <= SCAN_FUSE_MAX) {
+        k_scan_apply<OpSum, false, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, d_total);
+    } else {
+        k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, d_total);
+        PSK_LAUNCH_CHECK();
+        k_scan_apply<OpSum, false, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
+    }
     PSK_LAUNCH_CHECK();
     d.a->reset(mk);
 }
@@ -263,9 +285,13 @@ inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
     u32 *ta = d.a->get<u32>(nt);
     k_scan_reduce<OpMax><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
     PSK_LAUNCH_CHECK();
-    k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ta, nt, nullptr);
-    PSK_LAUNCH_CHECK();
-    k_scan_apply<OpMax, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta);
+    if (nt <= SCAN_FUSE_MAX) {
+        k_scan_apply<OpMax, true, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
+    } else {
+        k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ta, nt, nullptr);
+        PSK_LAUNCH_CHECK();
+        k_scan_apply<OpMax, true, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
+    }
     PSK_LAUNCH_CHECK();
     d.a->reset(mk);
 }
