@@ -161,11 +161,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             RelIn in{K, AA, B, false};
             k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, cnt);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tq, nt, cnt + 1);
-            if (lastconv) k_rmin_tiles<<<1, BLOCK, 0, d.s>>>(te, nt);
+            k_rel_tiles<<<lastconv ? 5 : 4, BLOCK, 0, d.s>>>(tg, ts, tk, tq, te, nt, cnt);
             PSK_LAUNCH_CHECK();
             const u32 *gl = (first && !byrank) ? nullptr : glab;
             // 28 B per item; the SA entries of the items that finish are

@@ -246,10 +246,7 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
             RelIn in{K, st.A, LB, true};
             k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(tg, nt, nullptr);
-            k_scan_tiles<OpMax><<<1, BLOCK, 0, d.s>>>(ts, nt, nullptr);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tk, nt, st.cnt);
-            k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(tq, nt, st.cnt + 1);
+            k_rel_tiles<<<4, BLOCK, 0, d.s>>>(tg, ts, tk, tq, te, nt, st.cnt);
             PSK_LAUNCH_CHECK();
             int pr = d.pbegin("lms_name_relabel", 28.0 * st.A);
             k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, FinishSink{st.L, st.F}, nxt, tg, ts, tk, te,

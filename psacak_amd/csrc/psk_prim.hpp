@@ -149,10 +149,10 @@ __global__ void k_scan_reduce(In in, u64 n, u32 *tile_agg) {
     if (threadIdx.x == 0) tile_agg[blockIdx.x] = agg;
 }
 
-// single workgroup: exclusive scan (Op) of nt tile aggregates in place;
-// *total (if non-null) receives the grand aggregate.
+// one workgroup: exclusive scan (Op) of nt tile aggregates in place; *total
+// (if non-null) receives the grand aggregate.
 template <class Op>
-__global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
+__device__ void scan_tiles_wg(u32 *a, u32 nt, u32 *total) {
     __shared__ u32 lds[NWAVES + 1];
     u32 carry = Op::id();
     for (u64 base = 0; base < nt; base += (u64)BLOCK * TSCAN_IPT) {
@@ -182,6 +182,11 @@ __global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
         carry = Op::op(carry, agg);
     }
     if (total && threadIdx.x == 0) *total = carry;
+}
+
+template <class Op>
+__global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
+    scan_tiles_wg<Op>(a, nt, total);
 }
 
 // One tile of SCAN_TILE items per workgroup: coalesced loads into LDS, each

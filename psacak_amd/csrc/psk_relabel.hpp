@@ -110,8 +110,8 @@ __global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *
     }
 }
 
-// single workgroup: a[i] = min(a[i+1 ..]) (exclusive suffix minimum)
-__global__ void k_rmin_tiles(u32 *a, u32 nt) {
+// one workgroup: a[i] = min(a[i+1 ..]) (exclusive suffix minimum)
+__device__ void rmin_tiles_wg(u32 *a, u32 nt) {
     __shared__ u32 wt[NWAVES];
     __shared__ u32 carry_sh;
     const int l = lane_id(), w = wave_id();
@@ -141,6 +141,21 @@ __global__ void k_rmin_tiles(u32 *a, u32 nt) {
             carry_sh = m;
         }
         __syncthreads();
+    }
+}
+
+// The tile scans of one relabel round in one launch, a workgroup each:
+// exclusive max of group starts (tg) and of subgroup starts (ts), exclusive
+// sums of kept items (tk, total to cnt[0]) and of kept subgroups (tq, total
+// to cnt[1]), and, with last-slot labels, the exclusive suffix minimum of
+// group ends (te).  Grid: 4 blocks, or 5 with te.
+__global__ void k_rel_tiles(u32 *tg, u32 *ts, u32 *tk, u32 *tq, u32 *te, u32 nt, u32 *cnt) {
+    switch (blockIdx.x) {
+        case 0: scan_tiles_wg<OpMax>(tg, nt, nullptr); break;
+        case 1: scan_tiles_wg<OpMax>(ts, nt, nullptr); break;
+        case 2: scan_tiles_wg<OpSum>(tk, nt, cnt); break;
+        case 3: scan_tiles_wg<OpSum>(tq, nt, cnt + 1); break;
+        default: rmin_tiles_wg(te, nt); break;
     }
 }
 
