@@ -67,11 +67,9 @@ __global__ void __launch_bounds__(BLOCK) k_place_lms_rec(const u32 *SA1, u32 m1,
     }
 }
 
-// The same placement without records: entry j of SA1 (an index into the
-// text-order LMS list) starts with the
-// byte c whose LMS range [lstart[c], lstart[c+1]) holds j (binary search in
-// LDS) and goes to j + lmsoff[c] as lms_text[SA1[j]], with the window of the
-// bytes before it (PW) for the induce.
+// The same placement without records (lean contexts): the entry goes to
+// j + lmsoff[c] as lms_text[SA1[j]], and its window (PW) is gathered from the
+// text: two dependent random reads instead of one.
 template <bool ALIGNED>
 __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, const u32 *lstart,
                                                      const u32 *lmsoff, const u32 *lms_text,
