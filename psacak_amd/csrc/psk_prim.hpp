@@ -328,13 +328,18 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
 constexpr int MS_ROUNDS = PSK_MS_ROUNDS;
 constexpr u64 MS_TILE = (u64)BLOCK * MS_ROUNDS;
 
-// LDS count of one digit per lane: up to 4 distinct digits of the wave are
-// added by one lane each (few distinct digits are the common case for byte
-// buckets), the rest with plain atomics.
+// LDS count of one digit per lane: up to LDS_COUNT_LEADERS distinct digits
+// of the wave are added by one lane each (few distinct digits are the common
+// case for byte buckets: a DNA bucket's predecessor bytes plus the "not
+// induced" digit are five), the rest with plain atomics.
+#ifndef PSK_LDS_LEADERS
+#define PSK_LDS_LEADERS 8
+#endif
+constexpr int LDS_COUNT_LEADERS = PSK_LDS_LEADERS;
 __device__ __forceinline__ void lds_count_wave(u32 *h, u32 x, bool valid) {
     bool pending = valid;
     #pragma unroll
-    for (int it = 0; it < 4; it++) {
+    for (int it = 0; it < LDS_COUNT_LEADERS; it++) {
         const u64 act = ballot(pending);
         if (!act) return;
         const int leader = __ffsll((long long)act) - 1;
