@@ -470,9 +470,12 @@ def main():
     sa_host = d_sa.cpu().numpy().view(np.uint32)
     pairs, bad_pairs = sampled_pair_check(text, sa_host)
 
-    # host-to-host rate (PCIe included): one extra call from host buffers
+    # host-to-host rate (PCIe included): one extra call from host buffers; the
+    # caller's SA buffer exists (touched) before the call, as psacak_inplace's
+    # suf does (lib.rs:77-82)
     log("host-to-host call")
     sa_h = np.empty(n, dtype=np.uint32)
+    sa_h.fill(0)
     th0 = time.perf_counter()
     P.psacak_inplace(text, sa_h, ctx)
     th = time.perf_counter() - th0
