@@ -115,7 +115,6 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     u64 *keys = d.a->get<u64>(A);
     u64 *kt = d.a->get<u64>(A);
     u32 *cnt = d.a->get<u32>(2);
-    PSK_HIP(hipMemcpyAsync(act, items, (u64)A * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
     const int B = bit_width(m);
     int Bh = B;
     const bool byrank = rank != nullptr && tails != nullptr && k > 0;
@@ -136,8 +135,16 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         const u32 *ac = act;
         const u32 *gi = gid;
         u32 *err = d.err;
+        u32 *acw = act;
         for_each(d, A, [=] __device__(u64 i) {
-            u32 p = ac[i];
+            // round 1 reads the caller's items and makes the working copy
+            u32 p;
+            if (first) {
+                p = items[i];
+                acw[i] = p;
+            } else {
+                p = ac[i];
+            }
             u64 q = (u64)p + hh;
             u32 lo = 0;
             if (q <= mm) lo = RK[q];
