@@ -334,12 +334,22 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     u32 k = 0;
     // a byte text's LMS substrings take few distinct values: try the
     // dictionary first there
-    bool dict = false;
-    if constexpr (std::is_same<RF, ByteRanks>::value)
-        dict = rt.dict_naming && name_lms_dict(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
-                                               rt.direct_rounds, ends);
-    if (!dict && !name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
-                                  rt.direct_rounds, ends)) {
+    bool named;
+    if constexpr (std::is_same<RF, ByteRanks>::value) {
+        // keys come out of the packed digit stream of the text, freed before
+        // the recursion
+        Arena::Mark mp = d.a->mark();
+        const PackedRanks pk = pack_ranks(d, rk, m, sigma);
+        named = (rt.dict_naming && name_lms_dict(d, pk, sigma, m, lms_text, m1, reduced, tails,
+                                                 &k, rt.direct_rounds, ends)) ||
+                name_lms_direct(d, pk, sigma, m, lms_text, m1, reduced, tails, &k,
+                                rt.direct_rounds, ends);
+        d.a->reset(mp);
+    } else {
+        named = name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
+                                rt.direct_rounds, ends);
+    }
+    if (!named) {
         d.a->reset(mk);
         return false;
     }

@@ -179,6 +179,132 @@ __device__ __forceinline__ u64 lms_key(RF rk, const u32 *lms_text, u32 m1, u64 m
     return (acc << 1) | (len < o + (u64)D ? 1ull : 0ull);
 }
 
+// Ranks of a byte text packed as key digits: the digit of position i
+// (rank(T[i]) + 1) takes Db bits at bit offset i*Db of the stream P, most
+// significant bit first, so the digits of consecutive positions read out of
+// one 64-bit window in key order.
+struct PackedRanks {
+    const u64 *P;
+    int Db;
+    // the 64 stream bits starting at the digit of position pos
+    __device__ __forceinline__ u64 window(u64 pos) const {
+        const u64 b = pos * (u64)Db;
+        const u64 w = b >> 6;
+        const u32 sh = (u32)(b & 63u);
+        u64 x = P[w] << sh;
+        if (sh) x |= P[w + 1] >> (64 - sh);
+        return x;
+    }
+    __device__ __forceinline__ u32 operator()(u64 pos) const {
+        return (u32)(window(pos) >> (64 - Db)) - 1u;
+    }
+};
+
+// lms_key from the packed stream: the up to D symbol digits of the window
+// come out of one 64-bit read (D*Db <= 62), instead of a lookup per symbol.
+__device__ __forceinline__ u64 lms_key(PackedRanks rk, const u32 *lms_text, u32 m1, u64 m, u32 r,
+                                       u64 o, int D, int Db, u64 term, const LmsEnds &E) {
+    bool isend;
+    u32 tid;
+    const u64 len = lms_span(lms_text, m1, m, r, E, isend, tid);
+    const u64 p = lms_text[r];
+    const u64 td = isend ? 0ull : term;
+    const u64 avail = len > o ? (len - o < (u64)D ? len - o : (u64)D) : 0ull;
+    u64 acc = 0;
+    if (avail)
+        acc = (rk.window(p + o) >> (64 - (int)avail * Db)) << (Db * (D - (int)avail));
+    // the terminator (or sentinel digit) at index len, if inside, then zeros
+    if (len >= o && len < o + (u64)D) acc |= td << (Db * (int)(o + (u64)D - 1 - len));
+    if (isend) acc = lms_end_index(acc, len, o, D, Db, tid, E);
+    return (acc << 1) | (len < o + (u64)D ? 1ull : 0ull);
+}
+
+// Packs positions [64t, 64t + 64) into the DB words P[DB*t ..]; positions at
+// or past n get digit 0.  The two words after the last thread's are zeroed
+// (a window read may touch them).  VEC: T is 16-byte aligned.
+template <int DB, bool VEC>
+__global__ void __launch_bounds__(BLOCK) k_pack_ranks(const u8 *T, u64 n, const u8 *crank, u64 *P,
+                                                      u64 nt) {
+    __shared__ u8 lr[256];
+    for (int i = threadIdx.x; i < 256; i += BLOCK) lr[i] = crank[i];
+    __syncthreads();
+    const u64 t = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    if (t == 0) {
+        P[nt * DB] = 0;
+        P[nt * DB + 1] = 0;
+    }
+    if (t >= nt) return;
+    const u64 base = t * 64;
+    u32 wv[16];
+    if (VEC && base + 64 <= n) {
+        const uint4 *q = (const uint4 *)(T + base);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint4 v = q[k];
+            wv[4 * k] = v.x;
+            wv[4 * k + 1] = v.y;
+            wv[4 * k + 2] = v.z;
+            wv[4 * k + 3] = v.w;
+        }
+    } else {
+        #pragma unroll
+        for (int k = 0; k < 16; k++) {
+            u32 x = 0;
+            #pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const u64 i = base + 4 * k + b;
+                if (i < n) x |= (u32)T[i] << (8 * b);
+            }
+            wv[k] = x;
+        }
+    }
+    u64 out[DB];
+    #pragma unroll
+    for (int k = 0; k < DB; k++) out[k] = 0;
+    #pragma unroll
+    for (int j = 0; j < 64; j++) {
+        const u64 dg = base + j < n ? (u64)lr[(wv[j >> 2] >> (8 * (j & 3))) & 255u] + 1 : 0ull;
+        const int b = j * DB, wd = b >> 6, off = b & 63;
+        if (off + DB <= 64) {
+            out[wd] |= dg << (64 - off - DB);
+        } else {
+            out[wd] |= dg >> (off + DB - 64);
+            out[wd + 1] |= dg << (128 - off - DB);
+        }
+    }
+    #pragma unroll
+    for (int k = 0; k < DB; k++) P[t * DB + k] = out[k];
+}
+
+// The packed digit stream of a byte text of n symbols (sigma distinct
+// bytes), in the arena: ceil(n/64)*Db + 2 words.
+inline PackedRanks pack_ranks(Dev &d, ByteRanks rk, u64 n, u64 sigma) {
+    const int Db = bit_width(sigma + 1);
+    const u64 nt = ceil_div(n, 64);
+    u64 *P = d.a->get<u64>(nt * (u64)Db + 2);
+    const bool vec = (((uintptr_t)rk.T) & 15u) == 0;
+    const u32 g = (u32)ceil_div(nt, BLOCK);
+#define PSK_PACK(DB)                                                                      \
+    case DB:                                                                              \
+        if (vec) k_pack_ranks<DB, true><<<g, BLOCK, 0, d.s>>>(rk.T, n, rk.crank, P, nt);  \
+        else k_pack_ranks<DB, false><<<g, BLOCK, 0, d.s>>>(rk.T, n, rk.crank, P, nt);     \
+        break;
+    switch (Db) {
+        PSK_PACK(2)
+        PSK_PACK(3)
+        PSK_PACK(4)
+        PSK_PACK(5)
+        PSK_PACK(6)
+        PSK_PACK(7)
+        PSK_PACK(8)
+        PSK_PACK(9)
+        default: fail(PSK_EDEVICE, "byte alphabet out of range");
+    }
+#undef PSK_PACK
+    PSK_LAUNCH_CHECK();
+    return PackedRanks{P, Db};
+}
+
 // Working state of the refinement rounds.  act[0..A): the substrings (their
 // text-order index r) still tied; gid[i]: dense id of item i's tie group,
 // whose label (first slot) is glab[gid]; L[r]: slot of a finished substring;
@@ -353,72 +479,23 @@ constexpr int DICT_LDS = 4096;     // per-workgroup cache of inserted keys
 constexpr int DICT_BLOCK = 1024;   // ranking workgroups (LDS histogram)
 constexpr u32 DICT_HIST = 16384;   // ranks per LDS histogram window (64 KB)
 constexpr u32 DICT_WINDOWS = 8;    // windows before the rest is counted globally
-constexpr int KEY_WORDS = 9;       // dwords that hold the <= 33 symbols of a key
 
 __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
     return (u32)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
-// lms_key (o = 0) of a byte text: the key's symbols are read with up to
-// KEY_WORDS aligned dword loads issued together (T must be 4-byte aligned),
-// ranked through an LDS copy of crank.  D <= 4*KEY_WORDS - 3.
-__device__ __forceinline__ u64 lms_key_bytes(const u8 *T, const u8 *lrank, const u32 *lms_text,
-                                             u32 m1, u64 m, u32 r, int D, int Db, u64 term,
-                                             const LmsEnds &E) {
-    bool isend;
-    u32 tid;
-    const u64 len = lms_span(lms_text, m1, m, r, E, isend, tid);
-    const u64 p = lms_text[r];
-    const u64 td = isend ? 0ull : term;
-    const u32 need = (u32)(len < (u64)D ? len : (u64)D);  // symbols in the key
-    const u32 sh = (u32)(p & 3u);
-    const u32 *W = (const u32 *)(T + (p - sh));
-    u32 w[KEY_WORDS];
-    #pragma unroll
-    for (int k = 0; k < KEY_WORDS; k++) w[k] = (u32)(4 * k) < sh + need ? W[k] : 0u;
-    u64 acc = 0;
-    #pragma unroll
-    for (int k = 0; k < KEY_WORDS; k++)
-        #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            int j = 4 * k + b - (int)sh;
-            if (j >= 0 && j < (int)need)
-                acc = (acc << Db) | ((u64)lrank[(w[k] >> (8 * b)) & 255u] + 1);
-        }
-    // digits need..D-1: the terminator at index len (if inside), then zeros
-    acc <<= Db * (D - (int)need);
-    if (len < (u64)D) acc |= td << (Db * (D - 1 - (int)len));
-    if (isend) acc = lms_end_index(acc, len, 0, D, Db, tid, E);
-    return (acc << 1) | (len < (u64)D ? 1ull : 0ull);
-}
-
-// Generic keys (unaligned text): one load per symbol.
-__device__ __forceinline__ u64 lms_key_any(const u8 *T, const u8 *lrank, const u32 *lms_text,
-                                           u32 m1, u64 m, u32 r, int D, int Db, u64 term,
-                                           const LmsEnds &E) {
-    struct R {
-        const u8 *T, *lr;
-        __device__ u32 operator()(u64 pos) const { return lr[T[pos]]; }
-    };
-    return lms_key(R{T, lrank}, lms_text, m1, m, r, 0, D, Db, term, E);
-}
-
 // Inserts every substring's key.  A workgroup remembers the keys it has
 // inserted in LDS (direct mapped), so a frequent key goes to the table once
 // per workgroup.  ctr[0] counts distinct keys, ctr[1] flags a full table.
-template <bool ALIGNED>
-__global__ void __launch_bounds__(BLOCK) k_dict_insert(ByteRanks rk, const u32 *lms_text, u32 m1,
-                                                       u64 m, int D, int Db, u64 term, LmsEnds E,
-                                                       u64 *tkey, u32 *ctr) {
+__global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32 *lms_text,
+                                                       u32 m1, u64 m, int D, int Db, u64 term,
+                                                       LmsEnds E, u64 *tkey, u32 *ctr) {
     __shared__ u64 seen[DICT_LDS];
-    __shared__ u8 lrank[256];
     for (int i = threadIdx.x; i < DICT_LDS; i += BLOCK) seen[i] = DICT_EMPTY;
-    lrank[threadIdx.x] = rk.crank[threadIdx.x];
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
     for (u64 r = (u64)blockIdx.x * BLOCK + threadIdx.x; r < m1; r += (u64)gridDim.x * BLOCK) {
-        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
-                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
+        u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
         u32 s = dict_hash(key, DICT_TBITS);
         u32 c = s & (DICT_LDS - 1);
         if (seen[c] == key) continue;
@@ -482,22 +559,18 @@ __device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cb
 // Dense rank of every substring's key into SL[r], and the number of
 // substrings per rank below DICT_HIST (an LDS histogram per workgroup).  A
 // key missing from the table sets ctr[1].
-template <bool ALIGNED>
-__global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(ByteRanks rk, const u32 *lms_text,
+__global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(PackedRanks rk, const u32 *lms_text,
                                                           u32 m1, u64 m, int D, int Db, u64 term,
                                                           LmsEnds E, const u64 *ckey, const u32 *crk,
                                                           int cbits, u32 K1, u32 *SL, u32 *count,
                                                           u32 *ctr) {
     __shared__ u32 h[DICT_HIST];
-    __shared__ u8 lrank[256];
     const u32 hn = K1 < DICT_HIST ? K1 : DICT_HIST;
     for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) h[i] = 0;
-    if (threadIdx.x < 256) lrank[threadIdx.x] = rk.crank[threadIdx.x];
     __syncthreads();
     for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
          r += (u64)gridDim.x * DICT_BLOCK) {
-        u64 key = ALIGNED ? lms_key_bytes(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E)
-                          : lms_key_any(rk.T, lrank, lms_text, m1, m, (u32)r, D, Db, term, E);
+        u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
         u32 dr = dict_find(ckey, crk, cbits, key);
         if (dr >= K1) {
             atomicOr(&ctr[1], 2u);
@@ -551,7 +624,7 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_count(const u32 *SL, u32 m1
 // contract as name_lms_direct.  Returns false, with nothing written, if the
 // keys take more than DICT_MAX_KEYS values or the rounds that follow hit
 // their limits (the caller then uses another strategy).
-inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
+inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *lms_text, u32 m1,
                           u32 *reduced, u32 *tails, u32 *k, int max_rounds,
                           LmsEnds E = LmsEnds{}) {
     if (m1 == 0) {
@@ -559,10 +632,9 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
         return true;
     }
     const int Db = bit_width(sigma + 1);
-    const int D = std::min(62 / Db, 4 * KEY_WORDS - 3);
-    if (D < 1) return false;
+    const int D = 62 / Db;
+    if (D < 1 || Db != rk.Db) return false;
     const u64 term = sigma + 1;
-    const bool aligned = (((uintptr_t)rk.T) & 3u) == 0;
     Arena::Mark mk = d.a->mark();
     const u64 tsize = 1ull << DICT_TBITS;
     u64 *tkey = d.a->get<u64>(tsize);
@@ -573,12 +645,8 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
     {
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 2048);
         int pr = d.pbegin("dict_insert", 12.0 * m1);
-        if (aligned)
-            k_dict_insert<true><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
-                                                               E, tkey, ctr);
-        else
-            k_dict_insert<false><<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term,
-                                                                E, tkey, ctr);
+        k_dict_insert<<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E, tkey,
+                                                      ctr);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
     }
@@ -617,12 +685,8 @@ inline bool name_lms_dict(Dev &d, ByteRanks rk, u64 sigma, u64 m, const u32 *lms
     {
         int pr = d.pbegin("dict_rank", 16.0 * m1);
         u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), 512);
-        if (aligned)
-            k_dict_rank<true><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(
-                rk, lms_text, m1, m, D, Db, term, E, ckey, crk, cbits, K1, SL, count, ctr);
-        else
-            k_dict_rank<false><<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(
-                rk, lms_text, m1, m, D, Db, term, E, ckey, crk, cbits, K1, SL, count, ctr);
+        k_dict_rank<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E,
+                                                         ckey, crk, cbits, K1, SL, count, ctr);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
         // the ranks above the first window, window by window from SL

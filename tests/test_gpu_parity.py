@@ -189,6 +189,29 @@ def test_dictionary_naming_cases(case):
     assert np.array_equal(got, oracle.sacak8(t))
 
 
+@pytest.mark.parametrize("sigma", [2, 3, 7, 15, 31, 63, 127, 255])
+def test_packed_digit_widths_and_offsets(ctx, sigma):
+    # level-0 keys come out of a stream of bit_width(sigma + 1) bits per
+    # byte (2..9 here); device texts at odd byte offsets take the packing
+    # path for unaligned texts; round 1 by dictionary and by sorting
+    import torch
+    rng = np.random.default_rng(sigma)
+    n = 150000
+    base = rng.integers(0, sigma, n + 8).astype(np.uint8)
+    base[n // 2:n // 2 + 3000] = base[1000:4000]
+    dt_all = torch.from_numpy(base).to("cuda:0")
+    with P.Context() as c:
+        for dict_naming in (True, False):
+            c.set_dict_naming(dict_naming)
+            for off in (0, 1, 3):
+                note(f"packed sigma={sigma} off={off} dict={dict_naming}")
+                t = base[off:off + n]
+                ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+                P.psacak8_dev(dt_all[off:off + n], ds, ctx=c)
+                torch.cuda.synchronize()
+                assert np.array_equal(ds.cpu().numpy().view(np.uint32), oracle.sacak8(t)), off
+
+
 @pytest.mark.parametrize("threshold", [0, 1, 512, 900, 1024])
 def test_doubling_levels(threshold):
     # recursion levels sorted by prefix doubling when distinct enough (1 =
