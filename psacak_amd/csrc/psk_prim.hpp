@@ -596,9 +596,120 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
     }
 }
 
+// Small sorts (n <= RS_SMALL_MAX): every pass in one workgroup and one
+// launch.  A pass counts all digits in LDS, then ranks the tiles in order as
+// k_rs_scatter does, carrying each digit's next output position from tile to
+// tile.  The buffers swap after every pass, as in radix_sort_kv.
+constexpr u64 RS_SMALL_MAX = 16 * RS_TILE;
+__global__ void __launch_bounds__(BLOCK) k_rs_small(u64 *k, u32 *v, u64 *kt, u32 *vt, u32 n,
+                                                    int passes) {
+    __shared__ u64 sk[RS_TILE];
+    __shared__ u32 sv[RS_TILE];
+    __shared__ u32 tstart[256], gbase[256], hist[256];
+    __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
+    __shared__ u32 red[NWAVES + 1];
+    const u32 dg = threadIdx.x;
+    const int w = wave_id();
+    const u32 nt = (u32)((n + RS_TILE - 1) / RS_TILE);
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = 8 * pass;
+        const u64 *ki = (pass & 1) ? kt : k;
+        const u32 *vi = (pass & 1) ? vt : v;
+        u64 *ko = (pass & 1) ? k : kt;
+        u32 *vo = (pass & 1) ? v : vt;
+        hist[dg] = 0;
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < n; i += BLOCK) atomicAdd(&hist[(u32)(ki[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        {
+            u32 tot;
+            gbase[dg] = block_exclusive_sum(hist[dg], red, &tot);  // includes a barrier
+        }
+        for (u32 t = 0; t < nt; t++) {
+            #pragma unroll
+            for (int r = 0; r < RS_ROUNDS; r++)
+                #pragma unroll
+                for (int w2 = 0; w2 < NWAVES; w2++) wcnt[r][w2][dg] = 0;
+            const u64 base = (u64)t * RS_TILE;
+            u64 key[RS_ROUNDS];
+            u32 val[RS_ROUNDS];
+            u32 dk[RS_ROUNDS];
+            unsigned short rk[RS_ROUNDS];
+            bool vv[RS_ROUNDS];
+            #pragma unroll
+            for (int r = 0; r < RS_ROUNDS; r++) {
+                u64 i = base + (u64)r * BLOCK + threadIdx.x;
+                vv[r] = i < n;
+                key[r] = vv[r] ? ki[i] : 0ull;
+                val[r] = vv[r] ? vi[i] : 0u;
+            }
+            __syncthreads();  // wcnt zeroed
+            #pragma unroll
+            for (int r = 0; r < RS_ROUNDS; r++) {
+                dk[r] = (u32)(key[r] >> shift) & 255u;
+                u64 peers = match_key8(vv[r], dk[r]);
+                rk[r] = (unsigned short)__popcll(peers & lanemask_lt());
+                if (vv[r] && rk[r] == 0) wcnt[r][w][dk[r]] = (unsigned short)__popcll(peers);
+            }
+            __syncthreads();
+            u32 c = 0;
+            #pragma unroll
+            for (int r = 0; r < RS_ROUNDS; r++)
+                #pragma unroll
+                for (int w2 = 0; w2 < NWAVES; w2++) c += wcnt[r][w2][dg];
+            u32 tot;
+            const u32 ts = block_exclusive_sum(c, red, &tot);  // includes a barrier
+            tstart[dg] = ts;
+            {
+                u32 run = ts;
+                #pragma unroll
+                for (int r = 0; r < RS_ROUNDS; r++)
+                    #pragma unroll
+                    for (int w2 = 0; w2 < NWAVES; w2++) {
+                        u32 x = wcnt[r][w2][dg];
+                        wcnt[r][w2][dg] = (unsigned short)run;
+                        run += x;
+                    }
+            }
+            __syncthreads();
+            #pragma unroll
+            for (int r = 0; r < RS_ROUNDS; r++) {
+                if (vv[r]) {
+                    u32 lp = (u32)wcnt[r][w][dk[r]] + rk[r];
+                    sk[lp] = key[r];
+                    sv[lp] = val[r];
+                }
+            }
+            __syncthreads();
+            const u32 cnt = n - base < RS_TILE ? (u32)(n - base) : (u32)RS_TILE;
+            for (u32 j = threadIdx.x; j < cnt; j += BLOCK) {
+                u64 kk = sk[j];
+                u32 d = (u32)(kk >> shift) & 255u;
+                u32 g = gbase[d] + (j - tstart[d]);
+                ko[g] = kk;
+                vo[g] = sv[j];
+            }
+            __syncthreads();  // sk, sv, tstart, gbase read
+            gbase[dg] += c;
+        }
+        // the pass's output is the next pass's input, in this workgroup
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
 inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int bits) {
     bool swapped = false;
     if (n == 0) return false;
+    if (n <= RS_SMALL_MAX) {
+        const int passes = (bits + 7) / 8;
+        if (passes == 0) return false;
+        int ps = d.pbegin("k_rs_small", 24.0 * (double)n * passes);
+        k_rs_small<<<1, BLOCK, 0, d.s>>>(k, v, kt, vt, (u32)n, passes);
+        PSK_LAUNCH_CHECK();
+        d.pend(ps);
+        return (passes & 1) != 0;
+    }
     Arena::Mark mk = d.a->mark();
     const u32 nt = ceil_div(n, RS_TILE);
     u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
