@@ -98,10 +98,14 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 // round 4 more than an eighth tied.
 // SA (optional): receives SA[label - 1] = p for every item as its label
 // becomes final, so the items' SA entries need no pass of their own.
+// runbits (optional, the S-type bits; only for a level's final L and S
+// sorts, items = every position of one type): labels that no later round and
+// no later sort reads are not written (ChainSink), using the items' run
+// offsets carried in their keys while the key bits leave room.
 inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        bool last_slot_labels, const u32 *rank = nullptr,
                        const u32 *tails = nullptr, u32 k = 0, bool giveup = false,
-                       u32 *SA = nullptr) {
+                       u32 *SA = nullptr, const u32 *runbits = nullptr) {
     Dev &d = rt.d;
     if (A == 0) return true;
     const u32 A0 = A;
@@ -128,6 +132,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     }
     u64 h = 1;
     int rounds = 0;
+    bool carried = false;  // the group ids hold run offsets
     while (A > 0) {
         rounds++;
         const bool first = rounds == 1;
@@ -136,6 +141,12 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         const u32 *gi = gid;
         u32 *err = d.err;
         u32 *acw = act;
+        // run offsets in this round's keys: the sorted bits end below them
+        // (at most 7 radix passes) and the group ids leave room for them
+        const bool cr = runbits && (first || carried) && 2 * h <= RUN_CAP && Bh + B <= 56 &&
+                        A < (1u << 28);
+        const bool prevc = carried, sit = last_slot_labels;
+        const u32 *rb = runbits;
         for_each(d, A, [=] __device__(u64 i) {
             // round 1 reads the caller's items and makes the working copy
             u32 p;
@@ -149,8 +160,13 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             u32 lo = 0;
             if (q <= mm) lo = RK[q];
             else atomicOr(err, 16u);
-            u32 hi = first ? (byrank ? rank[p] : RK[p]) : gi[i];
-            keys[i] = ((u64)hi << B) | (u64)lo;
+            const u32 gv = first ? 0u : gi[i];
+            u32 hi = first ? (byrank ? rank[p] : RK[p]) : (prevc ? gv & RUN_GID_MASK : gv);
+            u64 key = ((u64)hi << B) | (u64)lo;
+            if (cr)
+                key |= (u64)(first ? run_offset(rb, p, sit) : gv >> RUN_GID_SHIFT)
+                       << RUN_KEY_SHIFT;
+            keys[i] = key;
         });
         bool sw = radix_sort_kv(d, A, keys, act, kt, vt, Bh + B);
         const u64 *K = sw ? kt : keys;
@@ -165,7 +181,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
             u32 *tq = d.a->get<u32>(nt);
-            RelIn in{K, AA, B, false};
+            RelIn in{K, AA, B, false, cr ? (1ull << RUN_KEY_SHIFT) - 1 : ~0ull};
             k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
             k_rel_tiles<<<lastconv ? 5 : 4, BLOCK, 0, d.s>>>(tg, ts, tk, tq, te, nt, cnt);
@@ -174,7 +190,9 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             // 28 B per item; the SA entries of the items that finish are
             // added once their count is known
             pr = d.pbegin("chain_relabel", 28.0 * AA);
-            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, ChainSink{RK, lastconv, SA, mm, err}, nxt,
+            const ChainSink sink{RK, lastconv, SA, mm, err, cr ? (u32)(2 * h) : 0u,
+                                 last_slot_labels};
+            k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, sink, nxt,
                                                tg, ts, tk, te,
                                                tq, lastconv, mm, gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
@@ -189,6 +207,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         std::swap(gid, gid2);
         std::swap(glab, glab2);
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
+        carried = cr;
         h *= 2;
         if (h > m + 1 && A > 0) fail(PSK_EDEVICE, "chain sort did not converge");
         if (giveup && A > 0 &&
@@ -583,9 +602,9 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         d.a->reset(m2);
     }
     // every position is an L or an S item: the two sorts write the whole SA
-    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA);
+    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
     labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, sbits);
     d.a->reset(mk);
 }
 

@@ -501,7 +501,9 @@ __global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32
         if (seen[c] == key) continue;
         bool ok = false;
         for (u32 pr = 0; pr < DICT_MAX_PROBES; pr++) {
-            u64 cur = __hip_atomic_load(&tkey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // a slot changes once, from empty to its key: a plain load that
+            // still sees it empty is corrected by the CAS
+            u64 cur = tkey[s];
             if (cur == key) { ok = true; break; }
             if (cur == DICT_EMPTY) {
                 u64 old = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)DICT_EMPTY,
@@ -557,12 +559,13 @@ __device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cb
 }
 
 // Dense rank of every substring's key into SL[r], and the number of
-// substrings per rank below DICT_HIST (an LDS histogram per workgroup).  A
-// key missing from the table sets ctr[1].
+// substrings per rank below DICT_HIST: an LDS histogram per workgroup, written
+// out as row blockIdx.x of part (hn = min(K1, DICT_HIST) words per row) and
+// summed by k_dict_hist_sum.  A key missing from the table sets ctr[1].
 __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(PackedRanks rk, const u32 *lms_text,
                                                           u32 m1, u64 m, int D, int Db, u64 term,
                                                           LmsEnds E, const u64 *ckey, const u32 *crk,
-                                                          int cbits, u32 K1, u32 *SL, u32 *count,
+                                                          int cbits, u32 K1, u32 *SL, u32 *part,
                                                           u32 *ctr) {
     __shared__ u32 h[DICT_HIST];
     const u32 hn = K1 < DICT_HIST ? K1 : DICT_HIST;
@@ -580,8 +583,17 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(PackedRanks rk, const 
         if (dr < DICT_HIST) atomicAdd(&h[dr], 1u);
     }
     __syncthreads();
-    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK)
-        if (h[i]) atomicAdd(&count[i], h[i]);
+    u32 *row = part + (u64)blockIdx.x * hn;
+    for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) row[i] = h[i];
+}
+
+// count[j] = sum of the nb rows of part at column j, for j < hn.
+__global__ void k_dict_hist_sum(const u32 *part, u32 nb, u32 hn, u32 *count) {
+    const u32 j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= hn) return;
+    u32 acc = 0;
+    for (u32 b = 0; b < nb; b++) acc += part[(u64)b * hn + j];
+    count[j] = acc;
 }
 
 // Counts of the ranks in [lo, lo + DICT_HIST) from SL (LDS histogram); with
@@ -685,8 +697,12 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
     {
         int pr = d.pbegin("dict_rank", 16.0 * m1);
         u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), 512);
+        const u32 hn = std::min<u32>(K1, DICT_HIST);
+        u32 *part = d.a->get<u32>(blocks * hn);
         k_dict_rank<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E,
-                                                         ckey, crk, cbits, K1, SL, count, ctr);
+                                                         ckey, crk, cbits, K1, SL, part, ctr);
+        PSK_LAUNCH_CHECK();
+        k_dict_hist_sum<<<ceil_div(hn, BLOCK), BLOCK, 0, d.s>>>(part, (u32)blocks, hn, count);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
         // the ranks above the first window, window by window from SL

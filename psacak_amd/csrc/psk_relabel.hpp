@@ -23,11 +23,12 @@ struct RelIn {
     u64 A;
     int B;
     bool tbit;  // bit 0 of a key set: the item is finished (never kept)
+    u64 kmask = ~0ull;  // the sorted bits of a key (the rest rides along)
     // flags of item i: group start, subgroup start, kept, group end
     __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
-        u64 k = K[i], hi = k >> B;
-        u64 kp_ = i ? K[i - 1] : ~0ull;
-        u64 kn_ = i + 1 < A ? K[i + 1] : ~0ull;
+        u64 k = K[i] & kmask, hi = k >> B;
+        u64 kp_ = i ? K[i - 1] & kmask : ~0ull;
+        u64 kn_ = i + 1 < A ? K[i + 1] & kmask : ~0ull;
         gs = i == 0 || (kp_ >> B) != hi;
         ss = i == 0 || kp_ != k;
         ge = i + 1 == A || (kn_ >> B) != hi;
@@ -35,11 +36,38 @@ struct RelIn {
     }
 };
 
+// Run offsets.  In the final chain sorts of a level a position's label is
+// read by a later round only through an item of the same type run (chains
+// stay inside their run up to the terminal), at offset h' >= 2h from it, and
+// after the L sort only at the first L position of a run (the S chains'
+// terminals).  An item's run offset (same-type positions right before it,
+// capped at RUN_CAP) rides in the top bits of its key, above the sorted
+// bits, and of its group id between rounds.
+constexpr int RUN_KEY_SHIFT = 59;
+constexpr int RUN_GID_SHIFT = 27;
+constexpr u32 RUN_GID_MASK = (1u << RUN_GID_SHIFT) - 1u;
+constexpr u32 RUN_CAP = 31;
+
+// Run offset of position p in the type bits sb (bit set = S-type), for an
+// S item (s) or an L item; positions before 0 end a run.
+__device__ __forceinline__ u32 run_offset(const u32 *sb, u32 p, bool s) {
+    if (p == 0) return 0;
+    const u32 e = p - 1, we = e >> 5, be = e & 31u;
+    const u32 w0 = we ? sb[we - 1] : (s ? 0u : 0xFFFFFFFFu);
+    const u64 c = ((u64)sb[we] << 32) | w0;
+    const u32 y = (u32)(c >> (be + 1));  // bit 31 = position p-1, bit 0 = p-32
+    const u32 off = (u32)__clz(s ? ~y : y);
+    return off < RUN_CAP ? off : RUN_CAP;
+}
+
 // What a round does with an item's new label.  ChainSink: positions' labels
 // (the first subgroup of a group keeps its label, so it is not rewritten);
 // with SA, an item that is not kept (alone in its subgroup: its label is its
 // final slot + 1) also writes its SA entry p.  Items leave the round in
-// sorted order, so those writes land close together.
+// sorted order, so those writes land close together.  With run offsets in
+// the keys (kmin = 2h of the round, sitems: the S sort), a label no later
+// round reads is not written: an S item below offset kmin, an L item at an
+// offset in [1, kmin).
 // FinishSink: an item that is not kept records its subgroup's slot, and the
 // subgroup's first item marks that slot as a group start.
 struct ChainSink {
@@ -48,22 +76,38 @@ struct ChainSink {
     u32 *SA;
     u64 m;
     u32 *err;
-    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool first_sub, bool kept, bool) const {
-        if (lastconv || !first_sub) RK[p] = nl;
+    u32 kmin = 0;  // 0: every label is written
+    bool sitems = false;
+    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool first_sub, bool kept, bool,
+                                               u64 key) const {
+        if (lastconv || !first_sub) {
+            bool w = true;
+            if (kmin && kmin <= RUN_CAP) {
+                const u32 off = (u32)(key >> RUN_KEY_SHIFT);
+                w = sitems ? off >= kmin : (off == 0 || off >= kmin);
+            }
+            if (w) RK[p] = nl;
+        }
         if (SA && !kept) {
             if (nl >= 1 && nl <= m) SA[nl - 1] = p;
             else atomicOr(err, 64u);
         }
     }
+    // group id of a kept item for the next round (with its run offset)
+    __device__ __forceinline__ u32 gid(u32 q, u64 key) const {
+        return kmin ? q | ((u32)(key >> RUN_KEY_SHIFT) << RUN_GID_SHIFT) : q;
+    }
 };
 struct FinishSink {
     u32 *L, *F;
-    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool, bool kept, bool sstart) const {
+    __device__ __forceinline__ void operator()(u32 p, u32 nl, bool, bool kept, bool sstart,
+                                               u64) const {
         if (!kept) {
             L[p] = nl;
             if (sstart) F[nl] = 1;
         }
     }
+    __device__ __forceinline__ u32 gid(u32 q, u64) const { return q; }
 };
 
 constexpr int REL_ROUNDS = 16;
@@ -218,7 +262,8 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
         kk += rk;
         qq += rq;
         if (i < in.A) {
-            u64 hi = in.K[i] >> in.B;
+            const u64 kraw = in.K[i];
+            u64 hi = (kraw & in.kmask) >> in.B;
             u64 lab = glab ? (u64)glab[hi] : hi;
             if (lab > mm) {
                 atomicOr(err, 32u);
@@ -226,10 +271,10 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
                 u32 p = V[i];
                 u32 nl = lastconv ? (u32)lab - (g1sh[r * BLOCK + threadIdx.x] - s0)
                                   : (u32)lab + (s0 - g0);
-                sink(p, nl, s0 == g0, kp, ss);
+                sink(p, nl, s0 == g0, kp, ss, kraw);
                 if (kp) {
                     nxt[kk - 1] = p;
-                    gnx[kk - 1] = qq - 1;
+                    gnx[kk - 1] = sink.gid(qq - 1, kraw);
                     if (ss) glnx[qq - 1] = nl;
                 }
             }
