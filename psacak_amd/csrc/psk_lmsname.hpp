@@ -484,21 +484,27 @@ __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
     return (u32)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
-// Inserts every substring's key.  A workgroup remembers the keys it has
-// inserted in LDS (direct mapped), so a frequent key goes to the table once
-// per workgroup.  ctr[0] counts distinct keys, ctr[1] flags a full table.
+// Inserts every substring's key and records the table slot that holds it in
+// SLOT[r].  A workgroup remembers keys that sit in their home slot in LDS
+// (direct mapped), so a frequent key goes to the table once per workgroup
+// and its slot is its hash.  ctr[0] counts distinct keys, ctr[1] flags a full
+// table.
 __global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32 *lms_text,
                                                        u32 m1, u64 m, int D, int Db, u64 term,
-                                                       LmsEnds E, u64 *tkey, u32 *ctr) {
+                                                       LmsEnds E, u64 *tkey, u32 *ctr, u32 *SLOT) {
     __shared__ u64 seen[DICT_LDS];
     for (int i = threadIdx.x; i < DICT_LDS; i += BLOCK) seen[i] = DICT_EMPTY;
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
     for (u64 r = (u64)blockIdx.x * BLOCK + threadIdx.x; r < m1; r += (u64)gridDim.x * BLOCK) {
         u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
-        u32 s = dict_hash(key, DICT_TBITS);
-        u32 c = s & (DICT_LDS - 1);
-        if (seen[c] == key) continue;
+        const u32 home = dict_hash(key, DICT_TBITS);
+        const u32 c = home & (DICT_LDS - 1);
+        if (seen[c] == key) {
+            SLOT[r] = home;
+            continue;
+        }
+        u32 s = home;
         bool ok = false;
         for (u32 pr = 0; pr < DICT_MAX_PROBES; pr++) {
             // a slot changes once, from empty to its key: a plain load that
@@ -517,64 +523,30 @@ __global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32
             }
             s = (s + 1) & mask;
         }
-        if (ok) seen[c] = key;
-        else atomicOr(&ctr[1], 1u);
-    }
-}
-
-// Compact table of the K1 distinct keys (2^cbits >= 2*K1 slots) with their
-// dense ranks; every key is inserted once, so slots are claimed by CAS only.
-__global__ void k_dict_compact(const u64 *DK, u32 K1, int cbits, u64 *ckey, u32 *crk, u32 *ctr) {
-    const u32 mask = (1u << cbits) - 1u;
-    for (u64 j = (u64)blockIdx.x * BLOCK + threadIdx.x; j < K1; j += (u64)gridDim.x * BLOCK) {
-        u64 key = DK[j];
-        u32 s = dict_hash(key, cbits);
-        bool ok = false;
-        for (u32 pr = 0; pr <= mask; pr++) {
-            u64 old = atomicCAS((unsigned long long *)&ckey[s], (unsigned long long)DICT_EMPTY,
-                                (unsigned long long)key);
-            if (old == DICT_EMPTY) {
-                crk[s] = (u32)j;
-                ok = true;
-                break;
-            }
-            s = (s + 1) & mask;
+        if (ok) {
+            SLOT[r] = s;
+            if (s == home) seen[c] = key;
+        } else {
+            atomicOr(&ctr[1], 1u);
         }
-        if (!ok) atomicOr(&ctr[1], 8u);
     }
 }
 
-// Dense rank of a key in the compact table (a new kernel sees every insert,
-// so plain loads do); ~0u if absent.
-__device__ __forceinline__ u32 dict_find(const u64 *ckey, const u32 *crk, int cbits, u64 key) {
-    const u32 mask = (1u << cbits) - 1u;
-    u32 s = dict_hash(key, cbits);
-    for (u32 pr = 0; pr <= mask; pr++) {
-        u64 cur = ckey[s];
-        if (cur == key) return crk[s];
-        if (cur == DICT_EMPTY) return ~0u;
-        s = (s + 1) & mask;
-    }
-    return ~0u;
-}
-
-// Dense rank of every substring's key into SL[r], and the number of
-// substrings per rank below DICT_HIST: an LDS histogram per workgroup, written
-// out as row blockIdx.x of part (hn = min(K1, DICT_HIST) words per row) and
-// summed by k_dict_hist_sum.  A key missing from the table sets ctr[1].
-__global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(PackedRanks rk, const u32 *lms_text,
-                                                          u32 m1, u64 m, int D, int Db, u64 term,
-                                                          LmsEnds E, const u64 *ckey, const u32 *crk,
-                                                          int cbits, u32 K1, u32 *SL, u32 *part,
-                                                          u32 *ctr) {
+// Dense rank of every substring's key: SL[r] holds its key's table slot on
+// entry and receives ros[slot] (the rank of the key in that slot); the number
+// of substrings per rank below DICT_HIST goes to an LDS histogram per
+// workgroup, written out as row blockIdx.x of part (hn = min(K1, DICT_HIST)
+// words per row) and summed by k_dict_hist_sum.  A rank out of range sets
+// ctr[1].
+__global__ void __launch_bounds__(DICT_BLOCK) k_dict_rank(u32 *SL, u32 m1, const u32 *ros, u32 K1,
+                                                          u32 *part, u32 *ctr) {
     __shared__ u32 h[DICT_HIST];
     const u32 hn = K1 < DICT_HIST ? K1 : DICT_HIST;
     for (u32 i = threadIdx.x; i < hn; i += DICT_BLOCK) h[i] = 0;
     __syncthreads();
     for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
          r += (u64)gridDim.x * DICT_BLOCK) {
-        u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
-        u32 dr = dict_find(ckey, crk, cbits, key);
+        u32 dr = ros[SL[r] & ((1u << DICT_TBITS) - 1u)];
         if (dr >= K1) {
             atomicOr(&ctr[1], 2u);
             dr = 0;
@@ -654,11 +626,12 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
     d.zero(ctr, 4 * sizeof(u32));
     E.flag = ctr + 1;  // a text index that does not fit counts as a failure
     for_each(d, tsize, [=] __device__(u64 i) { tkey[i] = DICT_EMPTY; });
+    u32 *SL = d.a->get<u32>(m1);  // table slot, then dense rank, of every substring
     {
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 2048);
-        int pr = d.pbegin("dict_insert", 12.0 * m1);
+        int pr = d.pbegin("dict_insert", 16.0 * m1);
         k_dict_insert<<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E, tkey,
-                                                      ctr);
+                                                      ctr, SL);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
     }
@@ -669,7 +642,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
         d.a->reset(mk);
         return false;
     }
-    // distinct keys, sorted, then a compact table of (key, dense rank)
+    // distinct keys with their slots, sorted, then the rank of each used slot
     u64 *dk = d.a->get<u64>(K1), *dk2 = d.a->get<u64>(K1);
     u32 *ds = d.a->get<u32>(K1), *ds2 = d.a->get<u32>(K1);
     scan_exclusive(
@@ -683,24 +656,19 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
         nullptr);
     bool sw = radix_sort_kv(d, K1, dk, ds, dk2, ds2, 63);
     const u64 *DK = sw ? dk2 : dk;
-    const int cbits = std::max(10, bit_width((u64)K1 * 2 - 1));
-    u64 *ckey = d.a->get<u64>(1ull << cbits);
-    u32 *crk = d.a->get<u32>(1ull << cbits);
-    for_each(d, 1ull << cbits, [=] __device__(u64 i) { ckey[i] = DICT_EMPTY; });
-    k_dict_compact<<<std::min<u32>(ceil_div(K1, BLOCK), 4096), BLOCK, 0, d.s>>>(DK, K1, cbits, ckey,
-                                                                              crk, ctr);
-    PSK_LAUNCH_CHECK();
+    const u32 *DS = sw ? ds2 : ds;
+    // the insert table's keys are extracted: its memory holds ros[slot] now
+    u32 *ros = (u32 *)tkey;
+    for_each(d, K1, [=] __device__(u64 j) { ros[DS[j]] = (u32)j; });
     // rank and count every substring
-    u32 *SL = d.a->get<u32>(m1);
     u32 *count = d.a->get<u32>(K1 + 1);
     d.zero(count, ((u64)K1 + 1) * sizeof(u32));
     {
-        int pr = d.pbegin("dict_rank", 16.0 * m1);
+        int pr = d.pbegin("dict_rank", 8.0 * m1);
         u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), 512);
         const u32 hn = std::min<u32>(K1, DICT_HIST);
         u32 *part = d.a->get<u32>(blocks * hn);
-        k_dict_rank<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E,
-                                                         ckey, crk, cbits, K1, SL, part, ctr);
+        k_dict_rank<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(SL, m1, ros, K1, part, ctr);
         PSK_LAUNCH_CHECK();
         k_dict_hist_sum<<<ceil_div(hn, BLOCK), BLOCK, 0, d.s>>>(part, (u32)blocks, hn, count);
         PSK_LAUNCH_CHECK();
