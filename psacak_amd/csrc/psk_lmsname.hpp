@@ -475,8 +475,8 @@ constexpr u64 DICT_EMPTY = ~0ull;  // keys never have bit 63 set
 constexpr int DICT_TBITS = 22;     // insert table of 2^22 slots
 constexpr u32 DICT_MAX_KEYS = 1u << 21;
 constexpr u32 DICT_MAX_PROBES = 128;
-constexpr int DICT_LDS = 4096;     // per-workgroup cache of inserted keys
-constexpr int DICT_BLOCK = 1024;   // ranking workgroups (LDS histogram)
+constexpr int DICT_LDS = 8192;     // per-workgroup cache of inserted keys (two places each)
+constexpr int DICT_BLOCK = 1024;   // insert and ranking workgroups (64 KB of LDS each)
 constexpr u32 DICT_HIST = 16384;   // ranks per LDS histogram window (64 KB)
 constexpr u32 DICT_WINDOWS = 8;    // windows before the rest is counted globally
 
@@ -486,21 +486,25 @@ __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
 
 // Inserts every substring's key and records the table slot that holds it in
 // SLOT[r].  A workgroup remembers keys that sit in their home slot in LDS
-// (direct mapped), so a frequent key goes to the table once per workgroup
-// and its slot is its hash.  ctr[0] counts distinct keys, ctr[1] flags a full
-// table.
-__global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32 *lms_text,
-                                                       u32 m1, u64 m, int D, int Db, u64 term,
-                                                       LmsEnds E, u64 *tkey, u32 *ctr, u32 *SLOT) {
+// (each in one of two places), so a frequent key goes to the table once per
+// workgroup and its slot is its hash: a wave waits for the table whenever
+// one of its lanes misses, so the cache holds a whole byte text's
+// dictionary.  ctr[0] counts distinct keys, ctr[1] flags a full table.
+__global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, const u32 *lms_text,
+                                                            u32 m1, u64 m, int D, int Db, u64 term,
+                                                            LmsEnds E, u64 *tkey, u32 *ctr,
+                                                            u32 *SLOT) {
     __shared__ u64 seen[DICT_LDS];
-    for (int i = threadIdx.x; i < DICT_LDS; i += BLOCK) seen[i] = DICT_EMPTY;
+    for (int i = threadIdx.x; i < DICT_LDS; i += DICT_BLOCK) seen[i] = DICT_EMPTY;
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
-    for (u64 r = (u64)blockIdx.x * BLOCK + threadIdx.x; r < m1; r += (u64)gridDim.x * BLOCK) {
+    for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
+         r += (u64)gridDim.x * DICT_BLOCK) {
         u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
         const u32 home = dict_hash(key, DICT_TBITS);
-        const u32 c = home & (DICT_LDS - 1);
-        if (seen[c] == key) {
+        const u32 c1 = home & (DICT_LDS - 1);
+        const u32 c2 = (u32)((key * 0xC2B2AE3D27D4EB4Full) >> 51) & (DICT_LDS - 1);
+        if (seen[c1] == key || seen[c2] == key) {
             SLOT[r] = home;
             continue;
         }
@@ -525,7 +529,7 @@ __global__ void __launch_bounds__(BLOCK) k_dict_insert(PackedRanks rk, const u32
         }
         if (ok) {
             SLOT[r] = s;
-            if (s == home) seen[c] = key;
+            if (s == home) seen[seen[c1] == DICT_EMPTY ? c1 : c2] = key;
         } else {
             atomicOr(&ctr[1], 1u);
         }
@@ -628,10 +632,10 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
     for_each(d, tsize, [=] __device__(u64 i) { tkey[i] = DICT_EMPTY; });
     u32 *SL = d.a->get<u32>(m1);  // table slot, then dense rank, of every substring
     {
-        u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 2048);
+        u64 blocks = std::min<u64>(ceil_div(m1, DICT_BLOCK), 512);
         int pr = d.pbegin("dict_insert", 16.0 * m1);
-        k_dict_insert<<<(u32)blocks, BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E, tkey,
-                                                      ctr, SL);
+        k_dict_insert<<<(u32)blocks, DICT_BLOCK, 0, d.s>>>(rk, lms_text, m1, m, D, Db, term, E,
+                                                           tkey, ctr, SL);
         PSK_LAUNCH_CHECK();
         d.pend(pr);
     }
