@@ -133,6 +133,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     u64 h = 1;
     int rounds = 0;
     bool carried = false;  // the group ids hold run offsets
+    bool skipped = false;  // a round left labels unwritten
     while (A > 0) {
         rounds++;
         const bool first = rounds == 1;
@@ -191,7 +192,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             // added once their count is known
             pr = d.pbegin("chain_relabel", 28.0 * AA);
             const ChainSink sink{RK, lastconv, SA, mm, err, cr ? (u32)(2 * h) : 0u,
-                                 last_slot_labels};
+                                 last_slot_labels, !cr && skipped};
             k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, sink, nxt,
                                                tg, ts, tk, te,
                                                tq, lastconv, mm, gl, gid2, glab2, err);
@@ -208,6 +209,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         std::swap(glab, glab2);
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
         carried = cr;
+        skipped = skipped || cr;
         h *= 2;
         if (h > m + 1 && A > 0) fail(PSK_EDEVICE, "chain sort did not converge");
         if (giveup && A > 0 &&

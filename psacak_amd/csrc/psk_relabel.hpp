@@ -67,7 +67,10 @@ __device__ __forceinline__ u32 run_offset(const u32 *sb, u32 p, bool s) {
 // sorted order, so those writes land close together.  With run offsets in
 // the keys (kmin = 2h of the round, sitems: the S sort), a label no later
 // round reads is not written: an S item below offset kmin, an L item at an
-// offset in [1, kmin).
+// offset in [1, kmin); an L item at offset 0 is read only by the S sort,
+// which sees final labels, so it is written once, when it finishes (first
+// subgroup or not: earlier rounds skipped it).  wall: an earlier round
+// skipped writes and this one carries no offsets, so every label is written.
 // FinishSink: an item that is not kept records its subgroup's slot, and the
 // subgroup's first item marks that slot as a group start.
 struct ChainSink {
@@ -76,18 +79,20 @@ struct ChainSink {
     u32 *SA;
     u64 m;
     u32 *err;
-    u32 kmin = 0;  // 0: every label is written
+    u32 kmin = 0;  // 0: no run offsets in the keys
     bool sitems = false;
+    bool wall = false;
     __device__ __forceinline__ void operator()(u32 p, u32 nl, bool first_sub, bool kept, bool,
                                                u64 key) const {
-        if (lastconv || !first_sub) {
-            bool w = true;
-            if (kmin && kmin <= RUN_CAP) {
-                const u32 off = (u32)(key >> RUN_KEY_SHIFT);
-                w = sitems ? off >= kmin : (off == 0 || off >= kmin);
-            }
-            if (w) RK[p] = nl;
+        const bool changed = lastconv || !first_sub;
+        bool w;
+        if (kmin) {
+            const u32 off = (u32)(key >> RUN_KEY_SHIFT);
+            w = (!sitems && off == 0) ? !kept : (off >= kmin && changed);
+        } else {
+            w = changed || wall;
         }
+        if (w) RK[p] = nl;
         if (SA && !kept) {
             if (nl >= 1 && nl <= m) SA[nl - 1] = p;
             else atomicOr(err, 64u);
