@@ -599,8 +599,10 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
 // Small sorts (n <= RS_SMALL_MAX): every pass in one workgroup and one
 // launch.  A pass counts all digits in LDS, then ranks the tiles in order as
 // k_rs_scatter does, carrying each digit's next output position from tile to
-// tile.  The buffers swap after every pass, as in radix_sort_kv.
-constexpr u64 RS_SMALL_MAX = 16 * RS_TILE;
+// tile.  The buffers swap after every pass, as in radix_sort_kv.  Tiles run
+// one after another (a few microseconds each), so past a few tiles the
+// multi-workgroup passes win despite their four launches each.
+constexpr u64 RS_SMALL_MAX = 4 * RS_TILE;
 __global__ void __launch_bounds__(BLOCK) k_rs_small(u64 *k, u32 *v, u64 *kt, u32 *vt, u32 n,
                                                     int passes) {
     __shared__ u64 sk[RS_TILE];
