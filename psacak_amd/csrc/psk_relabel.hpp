@@ -118,8 +118,53 @@ struct FinishSink {
 constexpr int REL_ROUNDS = 16;
 constexpr u64 REL_TILE = (u64)BLOCK * REL_ROUNDS;
 
+// The four inclusive block scans of a relabel round at once (max of group
+// starts, max of subgroup starts, sum of kept items, sum of kept subgroups):
+// four independent wave scans and one LDS exchange, two barriers instead of
+// eight.  agg receives the four block aggregates.
+__device__ __forceinline__ void rel_scan4(u32 &g, u32 &s, u32 &k, u32 &q, u32 (*lds)[NWAVES],
+                                          u32 *agg) {
+    g = wave_incl<OpMax>(g);
+    s = wave_incl<OpMax>(s);
+    k = wave_incl<OpSum>(k);
+    q = wave_incl<OpSum>(q);
+    const int w = wave_id();
+    if (lane_id() == WAVE - 1) {
+        lds[0][w] = g;
+        lds[1][w] = s;
+        lds[2][w] = k;
+        lds[3][w] = q;
+    }
+    __syncthreads();
+    u32 pg = 0, ps = 0, pk = 0, pq = 0, ag = 0, as = 0, ak = 0, aq = 0;
+    #pragma unroll
+    for (int w2 = 0; w2 < NWAVES; w2++) {
+        const u32 x0 = lds[0][w2], x1 = lds[1][w2], x2 = lds[2][w2], x3 = lds[3][w2];
+        if (w2 < w) {
+            pg = pg > x0 ? pg : x0;
+            ps = ps > x1 ? ps : x1;
+            pk += x2;
+            pq += x3;
+        }
+        ag = ag > x0 ? ag : x0;
+        as = as > x1 ? as : x1;
+        ak += x2;
+        aq += x3;
+    }
+    __syncthreads();
+    g = g > pg ? g : pg;
+    s = s > ps ? s : ps;
+    k += pk;
+    q += pq;
+    agg[0] = ag;
+    agg[1] = as;
+    agg[2] = ak;
+    agg[3] = aq;
+}
+
 __global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *tq) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
+    __shared__ u32 l1[NWAVES + 1];
+    __shared__ u32 l4x[4][NWAVES];
     u64 base = (u64)blockIdx.x * REL_TILE;
     u32 g = 0, s = 0, k = 0, q = 0, e = 0xFFFFFFFFu;
     for (int r = 0; r < REL_ROUNDS; r++) {
@@ -134,11 +179,9 @@ __global__ void k_rel_reduce(RelIn in, u32 *tg, u32 *ts, u32 *tk, u32 *te, u32 *
             if (ge && (u32)i < e) e = (u32)i;
         }
     }
-    u32 a1, a2, a3, a4;
-    block_incl<OpMax>(g, l1, &a1);
-    block_incl<OpMax>(s, l2, &a2);
-    block_incl<OpSum>(k, l3, &a3);
-    block_incl<OpSum>(q, l4, &a4);
+    u32 ag4[4];
+    rel_scan4(g, s, k, q, l4x, ag4);
+    const u32 a1 = ag4[0], a2 = ag4[1], a3 = ag4[2], a4 = ag4[3];
     // min of group ends: minimum over the block
     u32 x = e;
     #pragma unroll
@@ -218,7 +261,7 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
                             const u32 *ts, const u32 *tk, const u32 *te, const u32 *tq,
                             bool lastconv, u64 mm, const u32 *glab, u32 *gnx, u32 *glnx,
                             u32 *err) {
-    __shared__ u32 l1[NWAVES + 1], l2[NWAVES + 1], l3[NWAVES + 1], l4[NWAVES + 1];
+    __shared__ u32 l4x[4][NWAVES];
     __shared__ u32 wt[NWAVES];
     __shared__ u32 g1sh[REL_TILE];
     const u64 base = (u64)blockIdx.x * REL_TILE;
@@ -257,11 +300,11 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
         bool gs = false, ss = false, kp = false, ge = false;
         if (i < in.A) in.flags(i, gs, ss, kp, ge);
-        u32 a1, a2, a3, a4;
-        u32 g0 = block_incl<OpMax>(gs ? (u32)i : 0u, l1, &a1);
-        u32 s0 = block_incl<OpMax>(ss ? (u32)i : 0u, l2, &a2);
-        u32 kk = block_incl<OpSum>(kp ? 1u : 0u, l3, &a3);
-        u32 qq = block_incl<OpSum>((kp && ss) ? 1u : 0u, l4, &a4);
+        u32 g0 = gs ? (u32)i : 0u, s0 = ss ? (u32)i : 0u;
+        u32 kk = kp ? 1u : 0u, qq = (kp && ss) ? 1u : 0u;
+        u32 ag4[4];
+        rel_scan4(g0, s0, kk, qq, l4x, ag4);
+        const u32 a1 = ag4[0], a2 = ag4[1], a3 = ag4[2], a4 = ag4[3];
         g0 = g0 > rg ? g0 : rg;
         s0 = s0 > rs ? s0 : rs;
         kk += rk;
