@@ -250,6 +250,24 @@ def test_psacak32_random_bucket_pointer_texts(ctx):
         assert np.array_equal(P.psacak32_raw(u, ctx), oracle.sacak32(tr))
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_psacak32_long_equal_runs(ctx, seed):
+    # long runs of one symbol keep the items of a type run tied for many
+    # rounds of a level's final chain sorts (the ties halve per round), so
+    # the sorts outlast the rounds whose keys carry run offsets and then
+    # write every label again; L runs (a run followed by a smaller symbol)
+    # and S runs (followed by a larger one) both occur
+    rng = np.random.default_rng(seed)
+    parts, total = [], 0
+    while total < 300000:
+        ln = int(rng.integers(1, 3000)) if rng.random() < 0.3 else int(rng.integers(1, 8))
+        parts.append(np.full(ln, int(rng.integers(0, 50)), dtype=np.uint32))
+        total += ln
+    u = np.concatenate(parts)[:300000]
+    note(f"psacak32 long runs seed={seed} n={len(u)}")
+    assert np.array_equal(P.psacak32_raw(u, ctx), oracle.sacak32_raw(u))
+
+
 def test_psacak32_raw_zipf(ctx):
     note("psacak32_raw zipf 1M")
     z = gen.zipf32(1 << 20, sigma=1 << 16, s=1.1, repeat_frac=0.2, seed=5)
