@@ -389,11 +389,45 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
         const u32 wm = wave_max(x);
         if (lane_id() == 0 && wm > __atomic_load_n(mx, __ATOMIC_RELAXED)) atomicMax(mx, wm);
     });
-    const int bits = std::max(1, bit_width(d.read_u32(mx)));
+    const u32 maxv = d.read_u32(mx);
+    const int bits = std::max(1, bit_width(maxv));
     bool sw = radix_sort_kv(d, n, k, v, kt, vt, bits);
     const u64 *K = sw ? kt : k;
     const u32 *V = sw ? vt : v;
     const u64 nn = n;
+    if ((u64)maxv < n) {
+        // symbols below n: first and last sorted index of each symbol in
+        // tables indexed by the symbol (written at the run boundaries of the
+        // sorted order), then every position looks its value up in text order,
+        // instead of a random scatter of the whole text
+        u32 *hd = first, *tl = d.a->get<u32>(n);
+        d.zero(ctr, sizeof(u32));
+        for_each_waves(d, n, [=] __device__(u64 i, bool live) {
+            bool st = false;
+            if (live) {
+                const u64 x = K[i];
+                st = i == 0 || K[i - 1] != x;
+                if (st) hd[x] = (u32)i;
+                if (i + 1 == nn || K[i + 1] != x) tl[x] = (u32)i;
+            }
+            const u64 b = ballot(st);
+            if (lane_id() == 0 && b) atomicAdd(ctr, (u32)__popcll(b));
+        });
+        const u32 distinct = d.read_u32(ctr);
+        if (distinct >= n) {
+            PSK_HIP(hipMemcpyAsync(SA, V, n * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
+            d.a->reset(mk);
+            return false;
+        }
+        u32 *sb = d.a->get<u32>((n + 31) / 32 + 1);
+        compute_types(d, t, n, sb);
+        for_each(d, nn, [=] __device__(u64 p) {
+            const u32 x = t[p];
+            out[p] = getbit(sb, p) ? tl[x] : hd[x];
+        });
+        d.a->reset(mk);
+        return true;
+    }
     // distinct count
     scan_exclusive(
         d, n, [=] __device__(u64 i) { return (u32)(i == 0 || K[i] != K[i - 1]); },
