@@ -268,7 +268,8 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
     Arena::Mark mk = d.a->mark();
     u32 nt = ceil_div(n, SCAN_TILE);
     u32 *ta = d.a->get<u32>(nt);
-    k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
+    // one tile: the fused apply reads no tile aggregates, so no reduce launch
+    if (nt > 1) k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
     PSK_LAUNCH_CHECK();
     if (nt <= SCAN_FUSE_MAX) {
         k_scan_apply<OpSum, false, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, d_total);
@@ -288,7 +289,7 @@ inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
     Arena::Mark mk = d.a->mark();
     u32 nt = ceil_div(n, SCAN_TILE);
     u32 *ta = d.a->get<u32>(nt);
-    k_scan_reduce<OpMax><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
+    if (nt > 1) k_scan_reduce<OpMax><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
     PSK_LAUNCH_CHECK();
     if (nt <= SCAN_FUSE_MAX) {
         k_scan_apply<OpMax, true, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
