@@ -519,6 +519,12 @@ __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, 
 // counts go to their own LDS slice, one pass per digit turns them into
 // tile-local offsets in (round, wave) order, and the tile is staged in LDS
 // in digit order before the coalesced write-out.
+// SMALL: off holds the unscanned counts (at most RS_FUSE_SCAN tiles, an
+// L2-resident matrix) and every workgroup derives its own offsets: the count
+// of its digit in the tiles before it plus the digit's start, a block scan of
+// the digits' totals.  No scan launches.
+constexpr u32 RS_FUSE_SCAN = 64;
+template <bool SMALL>
 __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki,
                                                       const u32 *__restrict__ vi,
                                                       u64 *__restrict__ ko, u32 *__restrict__ vo,
@@ -535,9 +541,23 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
         #pragma unroll
         for (int w = 0; w < NWAVES; w++) wcnt[r][w][dg] = 0;
     {
-        u64 o = (u64)dg * nt + t;
-        u32 g = off[o];
-        u32 c = off[o + 1] - g;
+        u32 g, c;
+        if (SMALL) {
+            const u32 *row = off + (u64)dg * nt;
+            u32 all = 0, pre = 0;
+            for (u32 t2 = 0; t2 < nt; t2++) {
+                const u32 x = row[t2];
+                all += x;
+                pre += t2 < t ? x : 0u;
+            }
+            c = row[t];
+            u32 sum;
+            g = block_exclusive_sum(all, red, &sum) + pre;  // includes barriers
+        } else {
+            u64 o = (u64)dg * nt + t;
+            g = off[o];
+            c = off[o + 1] - g;
+        }
         u32 tot;
         u32 ex = block_exclusive_sum(c, red, &tot);  // includes a barrier
         tstart[dg] = ex;
@@ -726,11 +746,14 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
         k_rs_count<<<ceil_div(nt, RS_CT), BLOCK, 0, d.s>>>(ki, n, shift, hist, nt);
         PSK_LAUNCH_CHECK();
         d.pend(pc);
-        scan_exclusive(
-            d, total, [=] __device__(u64 i) { return hist[i]; },
-            [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
+        const bool small = nt <= RS_FUSE_SCAN;
+        if (!small)
+            scan_exclusive(
+                d, total, [=] __device__(u64 i) { return hist[i]; },
+                [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
         int ps = d.pbegin("k_rs_scatter", 24.0 * n);
-        k_rs_scatter<<<nt, BLOCK, 0, d.s>>>(ki, vi, ko, vo, n, shift, hist, nt);
+        if (small) k_rs_scatter<true><<<nt, BLOCK, 0, d.s>>>(ki, vi, ko, vo, n, shift, hist, nt);
+        else k_rs_scatter<false><<<nt, BLOCK, 0, d.s>>>(ki, vi, ko, vo, n, shift, hist, nt);
         PSK_LAUNCH_CHECK();
         d.pend(ps);
         swapped = !swapped;
