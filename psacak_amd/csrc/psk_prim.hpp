@@ -255,8 +255,9 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, con
 }
 
 // scans of at most this many tiles combine the tile aggregates inside the
-// apply kernel (two launches instead of three)
-constexpr u32 SCAN_FUSE_MAX = 4096;
+// apply kernel (two launches instead of three; the single-workgroup tile scan
+// it saves costs ~24 us, the extra L2 reads of the largest scans less)
+constexpr u32 SCAN_FUSE_MAX = 8192;
 
 // Exclusive sum scan; d_total (device, may be null) receives the sum.
 template <class In, class Out>
