@@ -54,10 +54,13 @@ int psk_ctx_create(psk_ctx **out, uint64_t hbm_budget_bytes);
 void psk_ctx_destroy(psk_ctx *ctx);
 /* Message of the last failed call on this context ("" if none). */
 const char *psk_last_error(const psk_ctx *ctx);
-/* Release cached device workspace (keeps the stream). */
+/* Release cached device workspace (keeps the stream).  ctx NULL: the calling
+ * thread's default context (the one the entry points use for ctx NULL). */
 int psk_ctx_trim(psk_ctx *ctx);
 /* Device workspace the context holds now and the most it has held (bytes);
- * either pointer may be NULL.  No counterpart in the crate, which uses the
+ * either pointer may be NULL; ctx NULL: the calling thread's default context.
+ * A failed call leaves no workspace in use: the next call on the context
+ * starts from the same state.  No counterpart in the crate, which uses the
  * caller's SA plus O(block) buffers: for sizing HBM budgets. */
 int psk_ctx_workspace(const psk_ctx *ctx, uint64_t *reserved, uint64_t *peak);
 /* Naming strategy of this implementation (no counterpart in the crate, whose
@@ -179,7 +182,9 @@ int psk_server_submit_into(psk_server *s, const uint8_t *text, uint64_t n, uint3
  * *latency_ms = submit-to-completion time; the request is then released.
  * Returns the request's status (or PSK_ENOTFOUND). */
 int psk_server_poll(psk_server *s, uint64_t id, int *done, uint32_t *sa, double *latency_ms);
-/* Block until request id completes (timeout_ms < 0: no limit), then as poll. */
+/* Block until request id completes (timeout_ms < 0: no limit), then as poll.
+ * Several callers may poll or wait on one id at once: exactly one of them
+ * receives the result; the others return PSK_ENOTFOUND. */
 int psk_server_wait(psk_server *s, uint64_t id, uint32_t *sa, double *latency_ms,
                     double timeout_ms);
 /* Peak device bytes reserved by the server's contexts since creation. */

@@ -49,6 +49,10 @@ void ensure_ready(psk_ctx *c) {
     c->rt.d.a = &c->arena;
     c->rt.d.pinned = c->pinned;
     c->rt.d.prof = &c->prof;
+    {
+        const char *tr = getenv("PSACAK_AMD_TRACE");
+        c->rt.d.trace = tr && atoi(tr) > 0;
+    }
     c->ready = true;
 }
 
@@ -112,7 +116,7 @@ struct Timer {
 // every symbol < n? (bucket-pointer precondition, checked on the device)
 void check_precond32(psk_ctx *c, const u32 *d_text, u64 n) {
     Dev &d = c->rt.d;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *bad = d.a->get<u32>(1);
     d.zero(bad, sizeof(u32));
     const u64 nn = n;
@@ -120,7 +124,7 @@ void check_precond32(psk_ctx *c, const u32 *d_text, u64 n) {
         if (d_text[i] >= nn) atomicOr(bad, 1u);
     });
     u32 b = d.read_u32(bad);
-    d.a->reset(mk);
+    mk.reset();
     if (b) fail(PSK_EPRECOND, "psacak32: a symbol is >= n (not a bucket-pointer text)");
 }
 
@@ -134,10 +138,10 @@ void run32_dev(psk_ctx *c, const u32 *d_text, u64 n, u32 *d_sa, bool raw) {
     if (!raw) check_precond32(c, d_text, n);
     Timer t(c);
     if (raw) {
-        Arena::Mark mk = c->arena.mark();
+        ArenaScope mk(c->arena);
         u32 *tr = c->arena.get<u32>(n);
         if (translate32_run(c->rt, d_text, n, tr, d_sa)) psacak32_run(c->rt, tr, n, d_sa);
-        c->arena.reset(mk);
+        mk.reset();
     } else {
         psacak32_run(c->rt, d_text, n, d_sa);
     }
@@ -149,8 +153,12 @@ hipStream_t pick_stream(psk_ctx *c, void *stream) {
 }
 
 // Run f with the context's Dev pointed at `stream`.
+// Every device entry point runs through here: the arena scope returns the
+// context's workspace stack to where the call found it, on success and on
+// failure alike.
 template <class F>
 void on_stream(psk_ctx *c, void *stream, F f) {
+    ArenaScope outer(c->arena);
     hipStream_t saved = c->rt.d.s;
     c->rt.d.s = pick_stream(c, stream);
     hipStream_t saved_ctx = c->stream;
@@ -233,7 +241,7 @@ int psk_ctx_set_dict_naming(psk_ctx *ctx, int on) {
 }
 
 int psk_ctx_trim(psk_ctx *ctx) {
-    if (!ctx) return PSK_EINVAL;
+    if (!ctx) ctx = default_ctx();
     return guarded(ctx, [&] {
         if (ctx->ready) {
             PSK_HIP(hipStreamSynchronize(ctx->stream));
@@ -243,7 +251,7 @@ int psk_ctx_trim(psk_ctx *ctx) {
 }
 
 int psk_ctx_workspace(const psk_ctx *ctx, uint64_t *reserved, uint64_t *peak) {
-    if (!ctx) return PSK_EINVAL;
+    if (!ctx) ctx = default_ctx();
     if (reserved) *reserved = ctx->arena.reserved();
     if (peak) *peak = ctx->arena.peak();
     return PSK_OK;
@@ -285,18 +293,12 @@ int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *
     psk_ctx *c = ctx ? ctx : default_ctx();
     return guarded(c, [&] {
         ensure_ready(c);
-        Arena::Mark mk = c->arena.mark();
-        try {
-            u8 *dt = c->arena.get<u8>(n);
-            u32 *ds = c->arena.get<u32>(n);
-            upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n}}, dt);
-            run8_dev(c, dt, n, ds);
-            download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
-        } catch (...) {
-            c->arena.reset(mk);
-            throw;
-        }
-        c->arena.reset(mk);
+        ArenaScope mk(c->arena);
+        u8 *dt = c->arena.get<u8>(n);
+        u32 *ds = c->arena.get<u32>(n);
+        upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n}}, dt);
+        run8_dev(c, dt, n, ds);
+        download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
     });
 }
 
@@ -311,18 +313,12 @@ static int host32(psk_ctx *ctx, const uint32_t *text, uint64_t n, uint32_t *sa, 
     psk_ctx *c = ctx ? ctx : default_ctx();
     return guarded(c, [&] {
         ensure_ready(c);
-        Arena::Mark mk = c->arena.mark();
-        try {
-            u32 *dt = c->arena.get<u32>(n);
-            u32 *ds = c->arena.get<u32>(n);
-            upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n * sizeof(u32)}}, dt);
-            run32_dev(c, dt, n, ds, raw);
-            download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
-        } catch (...) {
-            c->arena.reset(mk);
-            throw;
-        }
-        c->arena.reset(mk);
+        ArenaScope mk(c->arena);
+        u32 *dt = c->arena.get<u32>(n);
+        u32 *ds = c->arena.get<u32>(n);
+        upload_segs(c->rt.d, c->xfer, {Seg{(char *)text, n * sizeof(u32)}}, dt);
+        run32_dev(c, dt, n, ds, raw);
+        download_segs(c->rt.d, c->xfer, ds, {Seg{(char *)sa, n * sizeof(u32)}});
     });
 }
 
@@ -452,24 +448,18 @@ int psk_batch8(psk_ctx *ctx, const uint8_t *const *texts, const uint64_t *lens,
             for (u64 i = g.first; i < g.second; i++) off[i - g.first + 1] = off[i - g.first] + lens[i];
             const u64 N = off.back(), K = g.second - g.first;
             if (N == 0) continue;
-            Arena::Mark mk = c->arena.mark();
-            try {
-                u8 *dt = c->arena.get<u8>(N);
-                u32 *ds = c->arena.get<u32>(N);
-                std::vector<Seg> tin, sout;
-                for (u64 i = 0; i < K; i++) {
-                    tin.push_back(Seg{(char *)texts[g.first + i], lens[g.first + i]});
-                    sout.push_back(Seg{(char *)sas[g.first + i], lens[g.first + i] * sizeof(u32)});
-                }
-                upload_segs(c->rt.d, c->xfer, tin, dt);
-                if (K == 1) psacak8_run(c->rt, dt, N, ds);
-                else batch8_run(c->rt, dt, off.data(), K, ds);
-                download_segs(c->rt.d, c->xfer, ds, sout);
-            } catch (...) {
-                c->arena.reset(mk);
-                throw;
+            ArenaScope mk(c->arena);
+            u8 *dt = c->arena.get<u8>(N);
+            u32 *ds = c->arena.get<u32>(N);
+            std::vector<Seg> tin, sout;
+            for (u64 i = 0; i < K; i++) {
+                tin.push_back(Seg{(char *)texts[g.first + i], lens[g.first + i]});
+                sout.push_back(Seg{(char *)sas[g.first + i], lens[g.first + i] * sizeof(u32)});
             }
-            c->arena.reset(mk);
+            upload_segs(c->rt.d, c->xfer, tin, dt);
+            if (K == 1) psacak8_run(c->rt, dt, N, ds);
+            else batch8_run(c->rt, dt, off.data(), K, ds);
+            download_segs(c->rt.d, c->xfer, ds, sout);
         }
         t.stop();
     });
@@ -517,7 +507,7 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
         ensure_ready(c);
         on_stream(c, stream, [&] {
             Dev &d = c->rt.d;
-            Arena::Mark mk = d.a->mark();
+            ArenaScope mk(*d.a);
             u32 *rank = d.a->get<u32>(n);
             u32 *bad = d.a->get<u32>(1);
             d.zero(bad, sizeof(u32));
@@ -544,7 +534,7 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
             });
             u32 b = d.read_u32(bad);
             *ok = b == 0;
-            d.a->reset(mk);
+            mk.reset();
         });
     });
 }
@@ -744,31 +734,42 @@ static int take_result(psk_server *s, std::unique_ptr<SrvReq> &r, uint32_t *sa, 
     if (lat) *lat = r->t_done - r->t_submit;
     int st = r->status;
     s->reqs.erase(r->id);
+    s->cv_done.notify_all();
     return st;
 }
 
 int psk_server_poll(psk_server *s, uint64_t id, int *done, uint32_t *sa, double *latency_ms) {
     if (!s || !done) return PSK_EINVAL;
+    *done = 0;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return PSK_ENOTFOUND;
     *done = it->second->done ? 1 : 0;
     if (!*done) return PSK_OK;
-    return take_result(s, it->second, sa, latency_ms);
+    int st = take_result(s, it->second, sa, latency_ms);
+    // waiters on the same id wake up and find it gone
+    s->cv_done.notify_all();
+    return st;
 }
 
 int psk_server_wait(psk_server *s, uint64_t id, uint32_t *sa, double *latency_ms,
                     double timeout_ms) {
     if (!s) return PSK_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return PSK_ENOTFOUND;
-    SrvReq *r = it->second.get();
-    auto ready = [&] { return r->done; };
+    if (s->reqs.find(id) == s->reqs.end()) return PSK_ENOTFOUND;
+    // the request is looked up again on every wake-up: a concurrent poll or
+    // wait on the same id may have taken its result (and released it) while
+    // this caller slept
+    auto ready = [&] {
+        auto it = s->reqs.find(id);
+        return it == s->reqs.end() || it->second->done;
+    };
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
     else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
         return PSK_EBUSY;
-    return take_result(s, s->reqs.find(id)->second, sa, latency_ms);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return PSK_ENOTFOUND;
+    return take_result(s, it->second, sa, latency_ms);
 }
 
 uint64_t psk_server_peak_bytes(psk_server *s) {

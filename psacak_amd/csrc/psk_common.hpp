@@ -115,7 +115,11 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T *lds, T *total) {
 
 // ---- device arena -------------------------------------------------------------
 // Stack allocator over large hipMalloc'd chunks that are kept for reuse.
-// mark()/reset() pop everything allocated after the mark.
+// mark()/reset() pop everything allocated after the mark; ArenaScope does it
+// when the scope ends, on every path out of it (exceptions included), so a
+// failed call leaves the arena where the call found it.
+// A fixed arena (attach) allocates from one caller-provided region and never
+// calls hipMalloc: an allocation that does not fit fails with PSK_EBUDGET.
 class Arena {
   public:
     struct Mark { size_t chunk, top; };
@@ -126,13 +130,34 @@ class Arena {
     // readable from other threads (a server sums its workers' arenas)
     u64 reserved() const { return reserved_.load(std::memory_order_relaxed); }
     u64 peak() const { return peak_; }
+    // highest stack top reached in a fixed arena (bytes from its base)
+    u64 fixed_peak() const { return fixed_peak_; }
+    void clear_fixed_peak() { fixed_peak_ = 0; }
 
     Mark mark() const { return {cur_, top_}; }
     void reset(Mark m) { cur_ = m.chunk; top_ = m.top; }
 
+    // allocate from [base, base + cap) only (device memory the caller owns)
+    void attach(void *base, u64 cap) {
+        release();
+        fixed_ = true;
+        chunks_.push_back({(char *)base, cap});
+        cur_ = top_ = 0;
+    }
+
     void *alloc(u64 bytes) {
         bytes = (bytes + 255) & ~(u64)255;
         if (bytes == 0) bytes = 256;
+        if (fixed_) {
+            if (chunks_.empty() || top_ + bytes > chunks_[0].cap)
+                fail(PSK_EBUDGET, "device workspace exceeds the fixed scratch region (" +
+                                      std::to_string(chunks_.empty() ? 0 : chunks_[0].cap) +
+                                      " bytes)");
+            void *p = chunks_[0].p + top_;
+            top_ += bytes;
+            fixed_peak_ = std::max(fixed_peak_, top_);
+            return p;
+        }
         while (cur_ < chunks_.size()) {
             if (top_ + bytes <= chunks_[cur_].cap) {
                 void *p = chunks_[cur_].p + top_;
@@ -166,10 +191,12 @@ class Arena {
     T *get(u64 count) { return (T *)alloc(count * sizeof(T)); }
 
     void release() {
-        for (auto &c : chunks_) (void)hipFree(c.p);
+        if (!fixed_)
+            for (auto &c : chunks_) (void)hipFree(c.p);
         chunks_.clear();
         reserved_.store(0, std::memory_order_relaxed);
         cur_ = top_ = 0;
+        fixed_ = false;
     }
 
   private:
@@ -180,6 +207,24 @@ class Arena {
     std::atomic<u64> reserved_{0};
     u64 peak_ = 0, budget_ = 0;
     u64 min_chunk_ = 64ull << 20;
+    bool fixed_ = false;
+    u64 fixed_peak_ = 0;
+};
+
+// Pops the arena back to where it stood at construction when the scope ends
+// (or at reset(), which may be called early; the destructor's reset is then a
+// no-op unless more was allocated since).
+class ArenaScope {
+  public:
+    explicit ArenaScope(Arena &a) : a_(a), m_(a.mark()) {}
+    ~ArenaScope() { a_.reset(m_); }
+    void reset() { a_.reset(m_); }
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+
+  private:
+    Arena &a_;
+    Arena::Mark m_;
 };
 
 // ---- kernel profiler ---------------------------------------------------------

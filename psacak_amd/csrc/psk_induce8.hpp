@@ -189,7 +189,7 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
     const u64 a = d.read_u32(d_start);
     u32 *err = d.err;
     if (a > limit) fail(PSK_EDEVICE, "queue seed count exceeds its bucket part");
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *run = d.a->get<u32>(a + 1);
     unsigned short *stop = d.a->get<unsigned short>(a + 1);  // byte | has << 8
     u32 *maxr = d_tmp;
@@ -200,7 +200,7 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
         const u64 aa = a;
         // one thread per seed; the grid is a multiple of the wave so every
         // lane takes part in the wave-wide maximum (one atomic per wave)
-        u64 blocks = (aa + BLOCK - 1) / BLOCK;
+        u64 blocks = std::min<u64>((aa + BLOCK - 1) / BLOCK, 65536u * 4);
         k_for_each_waves<<<(u32)std::max<u64>(blocks, 1), BLOCK, 0, d.s>>>(
             [=] __device__(u64 k, bool live) {
                 u32 r = 0;
@@ -242,7 +242,7 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
         PSK_LAUNCH_CHECK();
         d.pend(prec);
     }
-    if (nchain == 0) { d.a->reset(mk); return; }
+    if (nchain == 0) { mk.reset(); return; }
     // emission offsets
     u32 *eo = d.a->get<u32>(a + 1);
     u32 *tot = d_tmp + 1;
@@ -311,7 +311,7 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
             dig[base + i] = eb[j];
         });
     }
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // Per-bucket device tables: head, tail, fill (induce counters).

@@ -45,6 +45,7 @@ struct Runtime {
     // 0 = always recurse
     u32 doubling_distinct = 128;
     bool doubling_gave_up = false;  // in this call (reset by each entry point)
+    int depth = 0;                  // recursion level being solved (traces)
     // round 1 of direct naming at level 0 through a dictionary of the
     // distinct keys (false: by sorting every key)
     bool dict_naming = true;
@@ -110,7 +111,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     if (A == 0) return true;
     const u32 A0 = A;
     u32 Aprev = A;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *act = d.a->get<u32>(A);
     u32 *vt = d.a->get<u32>(A);
     u32 *gid = d.a->get<u32>(A), *gid2 = d.a->get<u32>(A);
@@ -177,7 +178,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         int pr = -1;
         {
             const bool lastconv = last_slot_labels && first;
-            Arena::Mark m3 = d.a->mark();
+            ArenaScope m3(*d.a);
             const u32 nt = ceil_div(AA, REL_TILE);
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
@@ -198,7 +199,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                                                tq, lastconv, mm, gl, gid2, glab2, err);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
-            d.a->reset(m3);
+            m3.reset();
         }
         u32 ag[2];
         d.read(ag, cnt, sizeof(ag));
@@ -207,6 +208,12 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         if (nxt != act) std::swap(act, vt);
         std::swap(gid, gid2);
         std::swap(glab, glab2);
+        if (d.trace) {
+            char what[64];
+            snprintf(what, sizeof(what), "level %d %s %s", rt.depth, giveup ? "doubling" : SA ? "final" : "stage1",
+                     last_slot_labels ? "S" : "L");
+            trace_round(d, what, rounds, AA, Bh + B, gid, ag[0], cr ? RUN_GID_MASK : ~0u, ag[1]);
+        }
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
         carried = cr;
         skipped = skipped || cr;
@@ -215,13 +222,13 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         if (giveup && A > 0 &&
             ((rounds == 2 && A > A0 / 16 && (u64)A * 4 > (u64)Aprev * 3) ||
              (rounds == 4 && A > A0 / 8))) {
-            d.a->reset(mk);
+            mk.reset();
             return false;
         }
         Aprev = A;
     }
     rt.max_rounds = std::max(rt.max_rounds, rounds);
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 
@@ -251,7 +258,7 @@ inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64
                                 u32 *RK, u32 *grp) {
     if (cnt == 0) return;
     u32 *err = d.err;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *vb = vals ? nullptr : d.a->get<u32>(cnt);
     // group ends: a wave compares neighbours by shuffle
     for_each_waves(d, cnt, [=] __device__(u64 j, bool live) {
@@ -269,7 +276,7 @@ inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64
         if (back <= v) RK[list[j]] = v - back + 1;
         else atomicOr(err, 256u);
     });
-    d.a->reset(mk);
+    mk.reset();
 }
 
 void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
@@ -306,7 +313,7 @@ inline void extract_lms_sorted(Dev &d, u64 m, const u32 *RK, const u32 *lms_text
 inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u32 k, u32 m1,
                             u32 *SA1, int depth) {
     Dev &d = rt.d;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     if (k < m1) {
         u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
         compute_types(d, reduced, m1, sb);
@@ -317,7 +324,7 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
     } else {
         inverse_names(d, reduced, m1, SA1);
     }
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // Induced path: names from the sorted LMS substrings, then the reduced problem.
@@ -327,7 +334,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
                           LevelStat &st, const u32 *EB = nullptr) {
     Dev &d = rt.d;
     if (m1 == 0) return;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *reduced = d.a->get<u32>(m1);
     u32 *tails = d.a->get<u32>(m1);
     u8 *diff = d.a->get<u8>(m1);
@@ -337,7 +344,7 @@ inline void solve_reduced(Runtime &rt, const C *T, const u32 *lms_sorted, const 
     st.k = k;
     rt.levels[depth] = st;
     recurse_reduced(rt, reduced, tails, k, m1, SA1, depth);
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // Direct path: names by sorting the LMS substrings, then the reduced problem.
@@ -349,7 +356,7 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     Dev &d = rt.d;
     if (m1 == 0) return true;
     if (rt.direct_rounds == 0) return false;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *reduced = d.a->get<u32>(m1);
     u32 *tails = d.a->get<u32>(m1);
     u32 k = 0;
@@ -359,25 +366,30 @@ inline bool solve_reduced_direct(Runtime &rt, RF rk, u64 sigma, u64 m, const u32
     if constexpr (std::is_same<RF, ByteRanks>::value) {
         // keys come out of the packed digit stream of the text, freed before
         // the recursion
-        Arena::Mark mp = d.a->mark();
+        ArenaScope mp(*d.a);
         const PackedRanks pk = pack_ranks(d, rk, m, sigma);
         named = (rt.dict_naming && name_lms_dict(d, pk, sigma, m, lms_text, m1, reduced, tails,
                                                  &k, rt.direct_rounds, ends)) ||
                 name_lms_direct(d, pk, sigma, m, lms_text, m1, reduced, tails, &k,
                                 rt.direct_rounds, ends);
-        d.a->reset(mp);
+        mp.reset();
     } else {
         named = name_lms_direct(d, rk, sigma, m, lms_text, m1, reduced, tails, &k,
                                 rt.direct_rounds, ends);
     }
     if (!named) {
-        d.a->reset(mk);
+        mk.reset();
         return false;
+    }
+    if (d.trace) {
+        char what[48];
+        snprintf(what, sizeof(what), "level %d naming", depth);
+        d.phase(what);
     }
     st.k = k;
     rt.levels[depth] = st;
     recurse_reduced(rt, reduced, tails, k, m1, SA1, depth);
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 
@@ -399,7 +411,7 @@ __device__ __forceinline__ u32 rep_hash8(const u32 *rank, u64 i) {
 }
 inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
     if (m < 4 * (u64)REP_WIN) return 0.0;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const u64 np = m - REP_WIN + 1;
     u32 *ctr = d.a->get<u32>(2);
     d.zero(ctr, 2 * sizeof(u32));
@@ -441,7 +453,7 @@ inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
             [=] __device__(u64, u32, u32) {}, dup);
         share = (double)d.read_u32(dup) / (double)na;
     }
-    d.a->reset(mk);
+    mk.reset();
     return share;
 }
 
@@ -459,7 +471,7 @@ inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
 inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const u32 *tails,
                              u32 k) {
     Dev &d = rt.d;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *RK = d.a->get<u32>(m + 1);
     u32 *items = d.a->get<u32>(m);
     const u64 mm = m;
@@ -473,10 +485,10 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const
         }
     });
     if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true, SA)) {
-        d.a->reset(mk);
+        mk.reset();
         return false;
     }
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 
@@ -484,6 +496,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
                     const u32 *rank, const u32 *tails, u32 k) {
     Dev &d = rt.d;
     if (m == 0) return;
+    rt.depth = depth;
     if (m <= 3) {
         k_naive_small<u32><<<1, 1, 0, d.s>>>(s, (u32)m, SA);
         PSK_LAUNCH_CHECK();
@@ -496,13 +509,18 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
             // the repeats that kept this level's suffixes tied recur below
             rt.doubling_gave_up = true;
         } else {
+            if (d.trace) {
+                char what[48];
+                snprintf(what, sizeof(what), "level %d doubling", depth);
+                d.phase(what);
+            }
             if ((int)rt.levels.size() <= depth) rt.levels.resize(depth + 1);
             rt.levels[depth] = LevelStat{m, 0, 0, k, 0, 4};
             rt.levels.resize(depth + 1);
             return;
         }
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const u64 nwords = (m + 31) / 32;
     u32 *lmsbits = d.a->get<u32>(nwords + 1);
     u32 *ctr = d.a->get<u32>(2);
@@ -545,6 +563,11 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     // by sorting the substrings: symbols ranked by the names of the level
     // above, else by their bucket-pointer values (order- and equality-
     // preserving: types inside an LMS substring follow from its symbols)
+    if (d.trace) {
+        char what[48];
+        snprintf(what, sizeof(what), "level %d types, LMS, item lists", depth);
+        d.phase(what);
+    }
     const bool named =
         rank ? solve_reduced_direct(rt, WordRanks{rank}, k, m, lms_text, m1, SA1, depth, st)
              : solve_reduced_direct(rt, WordRanks{s}, m, m, lms_text, m1, SA1, depth, st);
@@ -554,7 +577,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         labels_L(d, s, m, sbits, RK);
         {
             // LMS characters grouped by symbol (stable: text order inside a group)
-            Arena::Mark m2 = d.a->mark();
+            ArenaScope m2(*d.a);
             u64 *k0 = d.a->get<u64>(m1), *k1 = d.a->get<u64>(m1);
             u32 *v0 = d.a->get<u32>(m1), *v1 = d.a->get<u32>(m1);
             for_each(d, m1, [=] __device__(u64 r) {
@@ -563,7 +586,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
             });
             bool sw = radix_sort_kv(d, m1, k0, v0, k1, v1, bit_width(m));
             labels_lms_at_tails(d, s, sw ? v1 : v0, sw ? k1 : k0, m1, RK, cnt);
-            d.a->reset(m2);
+            m2.reset();
         }
         chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k);
         labels_S(d, s, m, sbits, RK);
@@ -575,13 +598,19 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     }
 
     // ---- final induce from sorted LMS suffixes (psacak32.rs:103-121) -------
+    rt.depth = depth;
+    if (d.trace) {
+        char what[48];
+        snprintf(what, sizeof(what), "level %d recursion below", depth);
+        d.phase(what);
+    }
     labels_L(d, s, m, sbits, RK);
     // sorted LMS suffixes are grouped by first symbol already
     if (m1) {
         // SA1 holds reduced indices: one 8-byte record (symbol, position) per
         // LMS position in text order (sequential reads), then one record
         // gather per sorted entry gives the list and its symbols
-        Arena::Mark m2 = d.a->mark();
+        ArenaScope m2(*d.a);
         // one block: R, vals, list
         u64 *R = d.a->get<u64>(2ull * m1 + m1 / 2 + 1);
         u64 *vals = R + m1;
@@ -601,13 +630,18 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
             vals[j] = r >> 32;
         });
         labels_lms_at_tails(d, s, list, vals, m1, RK, cnt);
-        d.a->reset(m2);
+        m2.reset();
     }
     // every position is an L or an S item: the two sorts write the whole SA
     chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
     labels_S(d, s, m, sbits, RK);
     chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, sbits);
-    d.a->reset(mk);
+    if (d.trace) {
+        char what[48];
+        snprintf(what, sizeof(what), "level %d final induce", depth);
+        d.phase(what);
+    }
+    mk.reset();
 }
 
 }  // namespace psk

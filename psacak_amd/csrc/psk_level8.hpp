@@ -109,7 +109,9 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         PSK_LAUNCH_CHECK();
         return;
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
+    rt.depth = 0;
+    d.phase(nullptr);
     const u64 nwords = (n + 31) / 32;
     u32 *sbits = d.a->get<u32>(nwords + 1);
     u32 *lmsbits = d.a->get<u32>(nwords + 1);
@@ -160,6 +162,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     u64 *PW = d.a->get<u64>(n);
     const Pred pr{T, SB, dbits};
 
+    d.phase("level 0 types, buckets, LMS list");
     LevelStat st{n, m1, nL_total, 0, 0, 1};
     rt.levels.resize(1);
     if (m1) {
@@ -223,7 +226,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         // psacak8.rs:35): LMS flags of the SA entries packed once, then compacted
         u32 *lms_sorted = d.a->get<u32>((u64)m1 + 1);
         {
-            Arena::Mark m2 = d.a->mark();
+            ArenaScope m2(*d.a);
             u32 *fl = d.a->get<u32>(nwords + 1);
             for_each_waves(d, n, [=] __device__(u64 i, bool live) {
                 u64 b = ballot(live && getbit(lmsbits, SA[i]));
@@ -242,12 +245,13 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
                     }
                 },
                 nullptr);
-            d.a->reset(m2);
+            m2.reset();
         }
         u32 *NP = d.a->get<u32>(n / 2 + 1);
         solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
     }
 
+    d.phase("level 0 records after recursion");
     // ---- sorted LMS suffixes to bucket tails (put_lmssuffixes psacak8.rs:76-87)
     // SA1 is sorted, so its entries come grouped by first byte: the bucket of
     // entry j follows from the LMS counts, without reading T[p]
@@ -280,8 +284,10 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         }
         PSK_LAUNCH_CHECK();
     }
+    d.phase("level 0 placement");
     induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);
-    d.a->reset(mk);
+    d.phase("level 0 final induce");
+    mk.reset();
 }
 
 // SAs of K texts stored back to back in T: text i is T[off[i] .. off[i+1])
@@ -293,7 +299,7 @@ inline void batch8_run(Runtime &rt, const u8 *T, const u64 *off, u64 K, u32 *SA)
     Dev &d = rt.d;
     const u64 N = off[K];
     if (N == 0) return;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const u64 nwords = (N + 31) / 32;
     u32 *EB = d.a->get<u32>(nwords + 1);
     u32 *SB = d.a->get<u32>(nwords + 1);
@@ -347,7 +353,7 @@ inline void batch8_run(Runtime &rt, const u8 *T, const u64 *off, u64 K, u32 *SA)
     });
     bool sw = radix_sort_kv(d, N, k0, v0, k1, v1, bit_width(K > 1 ? K - 1 : 0));
     PSK_HIP(hipMemcpyAsync(SA, sw ? v1 : v0, N * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // psacak32 on a bucket-pointer text (device buffers).
@@ -357,12 +363,12 @@ inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
     rt.max_rounds = 0;
     rt.doubling_gave_up = false;
     if (m == 0) return;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 *sb = d.a->get<u32>((m + 31) / 32 + 1);
     compute_types(d, s, m, sb);
     rt.levels.resize(1);
     level32(rt, s, m, sb, SA, 0);
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // translate_sample (psacak32.rs:912-943) on the device: rank the symbols,
@@ -370,7 +376,7 @@ inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
 // the SA directly) if every symbol is distinct.
 inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA) {
     Dev &d = rt.d;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u64 *k = d.a->get<u64>(n), *kt = d.a->get<u64>(n);
     u32 *v = d.a->get<u32>(n), *vt = d.a->get<u32>(n);
     u32 *first = d.a->get<u32>(n);
@@ -416,7 +422,7 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
         const u32 distinct = d.read_u32(ctr);
         if (distinct >= n) {
             PSK_HIP(hipMemcpyAsync(SA, V, n * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
-            d.a->reset(mk);
+            mk.reset();
             return false;
         }
         u32 *sb = d.a->get<u32>((n + 31) / 32 + 1);
@@ -425,7 +431,7 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
             const u32 x = t[p];
             out[p] = getbit(sb, p) ? tl[x] : hd[x];
         });
-        d.a->reset(mk);
+        mk.reset();
         return true;
     }
     // distinct count
@@ -435,7 +441,7 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
     u32 distinct = d.read_u32(ctr);
     if (distinct >= n) {
         PSK_HIP(hipMemcpyAsync(SA, V, n * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
-        d.a->reset(mk);
+        mk.reset();
         return false;
     }
     // head = first sorted index of the symbol's run; tail = last + 1
@@ -454,7 +460,7 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
         u32 p = V[i];
         out[p] = getbit(sb, p) ? last[first[i]] : first[i];
     });
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 

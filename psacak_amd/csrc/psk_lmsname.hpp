@@ -364,7 +364,7 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
         const u32 *V = sw ? st.vt : st.act;
         u32 *nxt = sw ? st.act : st.vt;
         {
-            Arena::Mark m3 = d.a->mark();
+            ArenaScope m3(*d.a);
             const u32 nt = ceil_div(st.A, REL_TILE);
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
@@ -380,14 +380,16 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
                                                st.gid2, st.glab2, err);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
-            d.a->reset(m3);
+            m3.reset();
         }
         u32 ag[2];
         d.read(ag, st.cnt, sizeof(ag));
+        const u64 before = st.A;
         st.A = ag[0];
         if (nxt != st.act) std::swap(st.act, st.vt);
         std::swap(st.gid, st.gid2);
         std::swap(st.glab, st.glab2);
+        trace_round(d, "lms names", st.rounds, before, st.Bg + LB, st.gid, st.A, ~0u, ag[1]);
         st.Bg = bit_width(ag[1] ? ag[1] - 1 : 0);
         st.off += (u64)D;
     }
@@ -426,7 +428,7 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
         *k = 0;
         return true;
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const int Db = bit_width(sigma + 1);
     u32 *hc = d.a->get<u32>(LEN_BINS);
     d.zero(hc, LEN_BINS * sizeof(u32));
@@ -451,14 +453,14 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
     }
     if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||
         (E.EB && d.read_u32(E.flag))) {
-        d.a->reset(mk);
+        mk.reset();
         return false;
     }
     const u32 kk = lms_slot_names(d, st, m1, tails);
     const u32 *F = st.F, *L = st.L;
     for_each(d, m1, [=] __device__(u64 r) { reduced[r] = F[L[r]]; });
     *k = kk;
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 
@@ -623,7 +625,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
     const int D = 62 / Db;
     if (D < 1 || Db != rk.Db) return false;
     const u64 term = sigma + 1;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const u64 tsize = 1ull << DICT_TBITS;
     u64 *tkey = d.a->get<u64>(tsize);
     u32 *ctr = d.a->get<u32>(4);
@@ -643,7 +645,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
     d.read(hc, ctr, sizeof(hc));
     const u32 K1 = hc[0];
     if (hc[1] || K1 > DICT_MAX_KEYS || K1 == 0) {
-        d.a->reset(mk);
+        mk.reset();
         return false;
     }
     // distinct keys with their slots, sorted, then the rank of each used slot
@@ -739,7 +741,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
         d.zero(E.flag, sizeof(u32));
         if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||
             (E.EB && d.read_u32(E.flag))) {
-            d.a->reset(mk);
+            mk.reset();
             return false;
         }
     }
@@ -754,7 +756,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
         reduced[r] = v != ~0u ? v : F[L[r]];
     });
     *k = kk;
-    d.a->reset(mk);
+    mk.reset();
     return true;
 }
 

@@ -8,6 +8,8 @@
 // in-tile order is the item order (the passes are stable).
 #pragma once
 
+#include <chrono>
+
 #include "psk_common.hpp"
 
 namespace psk {
@@ -52,6 +54,19 @@ struct Dev {
         if (bytes) PSK_HIP(hipMemsetAsync(d, 0, bytes, s));
     }
     void fill_u32(u32 *d, u32 v, u64 count);
+    // PSACAK_AMD_TRACE=1 in the environment: refinement rounds print their
+    // sizes to stderr (extra kernels and host syncs; diagnostics only)
+    bool trace = false;
+    // with trace: synchronise and print the time since the previous mark
+    double tmark = -1;
+    void phase(const char *name) {
+        if (!trace) return;
+        (void)hipStreamSynchronize(s);
+        const double t = (double)std::chrono::duration_cast<std::chrono::microseconds>(
+                             std::chrono::steady_clock::now().time_since_epoch()).count() / 1000.0;
+        if (name && tmark >= 0) fprintf(stderr, "[psk] phase %s: %.3f ms\n", name, t - tmark);
+        tmark = t;
+    }
 };
 
 template <typename F>
@@ -61,12 +76,13 @@ __global__ void k_for_each(F f, u64 n) {
     for (; i < n; i += stride) f(i);
 }
 
-// One item per thread, every thread of the grid calls f(i, i < n): for
-// kernels that use wave-wide operations.
+// Every thread of the grid calls f(i, i < n) for its items i (wave-uniform
+// trip count, so every lane reaches f's wave-wide operations): one item per
+// thread up to the grid's size, a grid-stride loop beyond it.
 template <typename F>
 __global__ void k_for_each_waves(F f, u64 n) {
-    u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x;
-    f(i, i < n);
+    const u64 stride = (u64)gridDim.x * BLOCK;
+    for (u64 base = (u64)blockIdx.x * BLOCK; base < n; base += stride) f(base + threadIdx.x, base + threadIdx.x < n);
 }
 
 // Launch f(i) for i in [0, n) with a grid-stride loop.
@@ -79,11 +95,14 @@ inline void for_each(Dev &d, u64 n, F f) {
     PSK_LAUNCH_CHECK();
 }
 
-// Launch f(i, i < n) with one thread per item (wave-uniform control flow).
+// Launch f(i, i < n) over [0, n) with wave-uniform control flow (grid capped
+// like for_each: texts of up to 2^32-2 bytes stay below the grid limit).
 template <typename F>
 inline void for_each_waves(Dev &d, u64 n, F f) {
     if (n == 0) return;
-    k_for_each_waves<<<ceil_div(n, BLOCK), BLOCK, 0, d.s>>>(f, n);
+    u64 blocks = (n + BLOCK - 1) / BLOCK;
+    if (blocks > 65536u * 4) blocks = 65536u * 4;
+    k_for_each_waves<<<(u32)blocks, BLOCK, 0, d.s>>>(f, n);
     PSK_LAUNCH_CHECK();
 }
 
@@ -266,7 +285,7 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
         if (d_total) d.zero(d_total, sizeof(u32));
         return;
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 nt = ceil_div(n, SCAN_TILE);
     u32 *ta = d.a->get<u32>(nt);
     // one tile: the fused apply reads no tile aggregates, so no reduce launch
@@ -280,14 +299,14 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
         k_scan_apply<OpSum, false, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
     }
     PSK_LAUNCH_CHECK();
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // Inclusive max scan.
 template <class In, class Out>
 inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
     if (n == 0) return;
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 nt = ceil_div(n, SCAN_TILE);
     u32 *ta = d.a->get<u32>(nt);
     if (nt > 1) k_scan_reduce<OpMax><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
@@ -300,7 +319,7 @@ inline void scan_inclusive_max(Dev &d, u64 n, In in, Out out) {
         k_scan_apply<OpMax, true, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
     }
     PSK_LAUNCH_CHECK();
-    d.a->reset(mk);
+    mk.reset();
 }
 
 // ---------------------------------------------------------------------------
@@ -423,7 +442,7 @@ inline int multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_c
         if (d_counts) d.zero(d_counts, 256 * sizeof(u32));
         return -1;
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u32 nt = ceil_div(n, MS_TILE);
     u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
     std::string tc = tag ? std::string(tag) + "_count" : std::string();
@@ -447,7 +466,7 @@ inline int multisplit2(Dev &d, u64 n, DigC dig_count, Dig dig, Put put, u32 *d_c
     k_ms_scatter<<<nt, BLOCK, 0, d.s>>>(dig, put, n, hist, nt);
     PSK_LAUNCH_CHECK();
     d.pend(ps);
-    d.a->reset(mk);
+    mk.reset();
     return ps;
 }
 
@@ -734,7 +753,7 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
         d.pend(ps);
         return (passes & 1) != 0;
     }
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     const u32 nt = ceil_div(n, RS_TILE);
     u32 *hist = d.a->get<u32>((u64)256 * nt + 1);
     const u64 total = (u64)256 * nt;
@@ -759,7 +778,7 @@ inline bool radix_sort_kv(Dev &d, u64 n, u64 *k, u32 *v, u64 *kt, u32 *vt, int b
         d.pend(ps);
         swapped = !swapped;
     }
-    d.a->reset(mk);
+    mk.reset();
     return swapped;
 }
 
@@ -767,6 +786,53 @@ inline int bit_width(u64 x) {
     int b = 0;
     while (x) { b++; x >>= 1; }
     return b;
+}
+
+// Trace line of a refinement round: items and groups before and after, and
+// the items left tied by the size of their group (log2 classes).  gid: group
+// id of every kept item (masked by gmask), contiguous groups.
+inline void trace_round(Dev &d, const char *what, int round, u64 items, int bits, const u32 *gid,
+                        u64 kept, u32 gmask, u64 groups) {
+    if (!d.trace) return;
+    constexpr int NC = 24;
+    u64 cls[NC] = {0};
+    if (kept && groups) {
+        ArenaScope mk(*d.a);
+        u32 *h = d.a->get<u32>(NC);
+        u32 *st = d.a->get<u32>(groups), *en = d.a->get<u32>(groups);
+        d.zero(h, NC * sizeof(u32));
+        const u64 kk = kept, gg = groups;
+        // first and last item of every group (dense ids), then the sizes
+        for_each(d, kk, [=] __device__(u64 i) {
+            const u32 g = gid[i] & gmask;
+            if (g >= gg) return;
+            if (!i || (gid[i - 1] & gmask) != g) st[g] = (u32)i;
+            if (i + 1 == kk || (gid[i + 1] & gmask) != g) en[g] = (u32)i;
+        });
+        // one atomic per class and wave (same-address atomics serialise)
+        k_for_each_waves<<<ceil_div(gg, BLOCK), BLOCK, 0, d.s>>>(
+            [=] __device__(u64 g, bool live) {
+                const u64 sz = live ? (u64)en[g] - st[g] + 1 : 0;
+                int c = 0;
+                while ((2ull << c) <= sz && c < NC - 1) c++;
+                for (int k = 0; k < NC; k++) {
+                    const u64 sum = wave_inclusive_sum<u64>(live && c == k ? sz : 0);
+                    if (lane_id() == WAVE - 1 && sum) atomicAdd(&h[k], (u32)sum);
+                }
+            },
+            gg);
+        PSK_LAUNCH_CHECK();
+        u32 hh[NC];
+        d.read(hh, h, sizeof(hh));
+        for (int c = 0; c < NC; c++) cls[c] = hh[c];
+        mk.reset();
+    }
+    fprintf(stderr, "[psk] %s round %d: %llu items, %d key bits (%d passes) -> %llu tied in %llu groups;",
+            what, round, (unsigned long long)items, bits, (bits + 7) / 8, (unsigned long long)kept,
+            (unsigned long long)groups);
+    for (int c = 0; c < NC; c++)
+        if (cls[c]) fprintf(stderr, " <%llu:%llu", 2ull << c, (unsigned long long)cls[c]);
+    fprintf(stderr, "\n");
 }
 
 }  // namespace psk

@@ -294,7 +294,7 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endb
     if (n == 0) return;
     u64 nwords = (n + 31) / 32;
     u32 nb = ceil_div(nwords, (u64)BLOCK * TY_WPT);
-    Arena::Mark mk = d.a->mark();
+    ArenaScope mk(*d.a);
     u8 *bh = d.a->get<u8>(nb);
     u8 *bf = d.a->get<u8>(nb);
     u8 *bc = d.a->get<u8>(nb);
@@ -314,7 +314,7 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endb
         PSK_LAUNCH_CHECK();
         run(MasksFromWords{Dw, Vw, nwords});
     }
-    d.a->reset(mk);
+    mk.reset();
 }
 
 __device__ __forceinline__ u32 getbit(const u32 *bits, u64 i) { return (bits[i >> 5] >> (i & 31)) & 1u; }

@@ -154,3 +154,45 @@ def test_tuned_defines_read_from_tuning_file(tmp_path, monkeypatch):
     f.write_text(json.dumps({"defines": {"PSK_RS_ROUNDS": 8, "PSK_MS_ROUNDS": 16}}))
     monkeypatch.setattr(build, "TUNING", str(f))
     assert sorted(build.tuned_defines()) == ["PSK_MS_ROUNDS=16", "PSK_RS_ROUNDS=8"]
+
+
+def test_server_concurrent_waits_on_one_id():
+    # two threads wait on the same request: exactly one receives its result
+    # (the SA, or the device error where there is no GPU), the other finds
+    # the request released (PSK_ENOTFOUND); neither reads freed state
+    import threading
+
+    with P.Server(workers=1) as s:
+        rid = s.submit(b"mississippi" * 50)
+        codes = []
+        lock = threading.Lock()
+
+        def waiter():
+            try:
+                s.wait(rid, timeout_ms=60000)
+                rc = P.PSK_OK
+            except P.PskError as e:
+                rc = e.code
+            with lock:
+                codes.append(rc)
+
+        ts = [threading.Thread(target=waiter) for _ in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert codes.count(P.PSK_ENOTFOUND) == 1, codes
+        assert set(codes) - {P.PSK_ENOTFOUND} <= {P.PSK_OK, P.PSK_EDEVICE}, codes
+        # poll after the release: unknown id
+        done = ctypes.c_int(0)
+        assert P._lib.psk_server_poll(s._h, rid, ctypes.byref(done), None, None) == P.PSK_ENOTFOUND
+        assert s.pending() == 0
+
+
+def test_default_context_trim_and_workspace():
+    # ctx NULL names the calling thread's default context, as in the entry points
+    r, p = ctypes.c_uint64(7), ctypes.c_uint64(7)
+    assert P._lib.psk_ctx_workspace(None, ctypes.byref(r), ctypes.byref(p)) == P.PSK_OK
+    assert P._lib.psk_ctx_trim(None) == P.PSK_OK
+    assert P._lib.psk_ctx_workspace(None, ctypes.byref(r), None) == P.PSK_OK
+    assert r.value == 0

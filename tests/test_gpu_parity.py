@@ -499,3 +499,41 @@ def test_budget_is_enforced():
         with pytest.raises(P.PskError) as e:
             P.psacak(gen.iid_dna(1 << 22), c)
         assert e.value.code in (P.PSK_EBUDGET, P.PSK_ENOMEM)
+
+
+def test_failed_call_leaves_workspace_reusable():
+    # a budget that runs out partway through a device call: the call fails
+    # with PSK_EBUDGET and leaves the context's workspace stack where it found
+    # it, so a smaller call on the same context succeeds inside the chunks
+    # already reserved (lib.rs:77-82: a call leaves no state behind)
+    import torch
+    note("budget failure then reuse")
+    dev = torch.device("cuda:0")
+    big = gen.markov_dna(32 << 20, seed=21)
+    small = gen.markov_dna(1 << 20, seed=22)
+    budget = 256 << 20
+    with P.Context(hbm_budget=budget) as c:
+        d_big = torch.from_numpy(big).to(dev)
+        s_big = torch.empty(len(big), dtype=torch.int32, device=dev)
+        with pytest.raises(P.PskError) as e:
+            P.psacak8_dev(d_big, s_big, ctx=c)
+        assert e.value.code == P.PSK_EBUDGET
+        r1, _ = c.workspace()
+        assert 0 < r1 <= budget
+        d_small = torch.from_numpy(small).to(dev)
+        s_small = torch.empty(len(small), dtype=torch.int32, device=dev)
+        exp = oracle.sacak8(small)
+        for _ in range(2):
+            P.psacak8_dev(d_small, s_small, ctx=c)
+            torch.cuda.synchronize()
+            assert np.array_equal(s_small.cpu().numpy().view(np.uint32), exp)
+            r2, _ = c.workspace()
+            assert r2 == r1, (r1, r2)
+        # the same through the host entry point and psacak32_raw
+        with pytest.raises(P.PskError) as e:
+            P.psacak(big, c)
+        assert e.value.code == P.PSK_EBUDGET
+        assert np.array_equal(P.psacak(small, c), exp)
+        z = gen.zipf32(1 << 16, sigma=1 << 12, seed=4)
+        assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
+        assert c.workspace()[0] <= budget
