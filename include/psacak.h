@@ -122,6 +122,22 @@ int psk_psacak32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t 
 int psk_psacak32_raw_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, uint32_t *d_sa,
                          void *stream);
 
+/* ---- zero-allocation device entry point ---------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): psk_psacak8_dev on the current device without a context,
+ * taking all of its device workspace from a fixed array of
+ * PSK_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * (until the process ends).  The call makes no hipMalloc/hipFree or other
+ * allocating call; it enqueues on `stream` (NULL: the null stream) and
+ * returns when the SA is complete.  Texts longer than PSK_NOALLOC_MAX_N
+ * bytes return PSK_ETOOLONG.  Calls on one GPU share its array: they are
+ * serialised by a lock, so calls that would overlap on one GPU run one
+ * after another. */
+#define PSK_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define PSK_NOALLOC_MAX_N (2ull << 20)
+int psk_psacak8_dev_noalloc(const uint8_t *d_text, uint64_t n, uint32_t *d_sa, void *stream);
+
 /* Per-level statistics of the last psacak8/psacak32 call on ctx:
  * rows of {m, m1, mL, k, sum of LMS-substring lengths, symbol bytes}.
  * Returns the number of levels written (<= max_levels). */

@@ -25,7 +25,7 @@ __all__ = [
     "PskError", "Context", "psacak", "psacak_inplace", "psacak32", "psacak32_raw",
     "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "batch8", "batch8_dev",
     "workspace_bytes8", "workspace_bytes32", "lib_path", "version", "kernel_profile",
-    "kernel_stats", "Server",
+    "kernel_stats", "Server", "psacak8_dev_noalloc",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -97,6 +97,7 @@ def _load():
         "psk_server_pending": ([vp], u64),
         "psk_server_destroy": ([vp], None),
         "psk_profile_stats": ([vp, ctypes.c_char_p, u64, vp, vp, vp, i32], i32),
+        "psk_psacak8_dev_noalloc": ([vp, u64, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -318,6 +319,21 @@ def psacak32_raw_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = N
     rc = _lib.psk_psacak32_raw_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
                                    _stream_ptr(stream))
     _raise(rc, ctx)
+
+
+# psk_psacak8_dev_noalloc limits (include/psacak.h)
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_N = 2 << 20
+
+
+def psacak8_dev_noalloc(d_text, d_sa, n: int | None = None, stream=None):
+    """psacak8_dev without a context and without any device allocation: the
+    workspace comes from the library's fixed per-GPU scratch array (texts of
+    at most NOALLOC_MAX_N bytes)."""
+    if n is None:
+        n = d_text.numel()
+    rc = _lib.psk_psacak8_dev_noalloc(_dev_ptr(d_text), n, _dev_ptr(d_sa), _stream_ptr(stream))
+    _raise(rc)
 
 
 def check8_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,

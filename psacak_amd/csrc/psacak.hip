@@ -176,6 +176,51 @@ void on_stream(psk_ctx *c, void *stream, F f) {
 
 }  // namespace
 
+// ---- zero-allocation entry point ----------------------------------------------
+// psk_psacak8_dev_noalloc takes every byte of device workspace from one
+// file-scope array (each GPU the library runs on holds its own copy until the
+// process ends) instead of the context arenas' hipMalloc'd chunks.  The
+// array's name is a host-side stand-in for the device symbol: host code only
+// ever uses the per-device address that hipGetSymbolAddress returned for it.
+static __device__ __align__(256) unsigned char scratch[PSK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+constexpr u64 kNoAllocFlagBytes = 256;  // internal index-check flag, then the workspace stack
+
+struct NoAllocState {
+    std::mutex mu;          // one call at a time per device (they share the array)
+    bool ready = false;
+    void *scratch = nullptr;  // this device's copy of the array
+    Arena arena;            // fixed: allocates from the array only
+    Runtime rt;
+    u32 host_buf[4096];     // host staging of small readbacks (pageable)
+};
+
+NoAllocState g_noalloc[kNoAllocMaxDevices];
+
+// Set-up on the first call on a device, with that device current.
+void noalloc_setup(NoAllocState &g) {
+    if (g.ready) return;
+    void *p = nullptr;
+    PSK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g.scratch = p;
+    g.arena.attach((char *)p + kNoAllocFlagBytes, PSK_NOALLOC_SCRATCH_BYTES - kNoAllocFlagBytes);
+    g.rt.d.a = &g.arena;
+    g.rt.d.err = (u32 *)p;
+    g.rt.d.pinned = g.host_buf;
+    g.rt.d.prof = nullptr;
+    // per-call workspace proportional to the text: no fixed-size naming
+    // dictionary, no placement records (results are identical)
+    g.rt.dict_naming = false;
+    g.rt.lms_records = false;
+    g.rt.levels.reserve(64);
+    g.ready = true;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *psk_strerror(int status) {
@@ -536,6 +581,33 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
             *ok = b == 0;
             mk.reset();
         });
+    });
+}
+
+// ---- zero-allocation entry point (state above the C ABI block) -------------
+int psk_psacak8_dev_noalloc(const uint8_t *d_text, uint64_t n, uint32_t *d_sa,
+                                       void *stream) {
+    int rc = check8(d_text, n, d_sa, n);
+    if (rc != PSK_OK || n == 0) return rc;
+    if (n > PSK_NOALLOC_MAX_N) return PSK_ETOOLONG;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return PSK_EDEVICE;
+    }
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return PSK_EDEVICE;
+    NoAllocState &g = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(g.mu);
+    return guarded(nullptr, [&] {
+        noalloc_setup(g);
+        Dev &d = g.rt.d;
+        d.s = (hipStream_t)stream;
+        ArenaScope sc(g.arena);
+        // the array keeps the last call's contents: the flag starts at zero
+        d.zero(d.err, sizeof(u32));
+        psacak8_run(g.rt, d_text, n, d_sa);
+        const u32 e = d.read_u32(d.err);
+        if (e) fail(PSK_EDEVICE, "internal index check failed (flags " + std::to_string(e) + ")");
     });
 }
 
