@@ -143,6 +143,19 @@ int psk_psacak8_dev_noalloc(const uint8_t *d_text, uint64_t n, uint32_t *d_sa, v
  * Returns the number of levels written (<= max_levels). */
 int psk_last_levels(const psk_ctx *ctx, uint64_t *rows, int max_levels);
 
+/* Capture of an intermediate, for parity tests against a CPU restatement (no
+ * counterpart in the crate): with level >= 0, the following calls on ctx copy
+ * the translated reduced text of that recursion level (bucket-pointer form of
+ * the names of its LMS substrings, LMS positions in text order: the next
+ * level's text, naming.rs:263-274) to host memory of the context; -1 (default)
+ * turns it off.  PSK_EINVAL outside [-1, 63]. */
+int psk_ctx_set_capture(psk_ctx *ctx, int level);
+/* The text captured by the last call: *count entries, copied to out if out
+ * is non-NULL (cap entries available, PSK_EINVAL if too few).  PSK_ENOTFOUND
+ * if the last call did not name that level (it was sorted otherwise, or the
+ * recursion ended above it). */
+int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *count);
+
 /* Time (ms, HIP events) of the last call's device work, by phase:
  * [0] total, [1] level-0 induce, [2] deeper levels, [3] naming+typing. */
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
@@ -174,6 +187,9 @@ int psk_batch8_dev(psk_ctx *ctx, const uint8_t *d_texts, const uint64_t *offsets
  * Burkhardt-Kaerkkaeinen adjacent-pair condition).  *ok = 1 if valid. */
 int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32_t *d_sa,
                    int *ok, void *stream);
+/* The same check for an integer text (u32 symbols; n <= 2^31-1). */
+int psk_check32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, const uint32_t *d_sa,
+                    int *ok, void *stream);
 /* ---- serving ------------------------------------------------------------ */
 /* No counterpart in the reference (a library call only).  A server runs
  * max_workers worker threads, each with its own context and stream.  Requests

@@ -23,7 +23,8 @@ import numpy as np
 
 __all__ = [
     "PskError", "Context", "psacak", "psacak_inplace", "psacak32", "psacak32_raw",
-    "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "batch8", "batch8_dev",
+    "psacak8_dev", "psacak32_dev", "psacak32_raw_dev", "check8_dev", "check32_dev", "batch8",
+    "batch8_dev",
     "workspace_bytes8", "workspace_bytes32", "lib_path", "version", "kernel_profile",
     "kernel_stats", "Server", "psacak8_dev_noalloc",
 ]
@@ -98,6 +99,9 @@ def _load():
         "psk_server_destroy": ([vp], None),
         "psk_profile_stats": ([vp, ctypes.c_char_p, u64, vp, vp, vp, i32], i32),
         "psk_psacak8_dev_noalloc": ([vp, u64, vp, vp], i32),
+        "psk_check32_dev": ([vp, vp, u64, vp, ctypes.POINTER(i32), vp], i32),
+        "psk_ctx_set_capture": ([vp, i32], i32),
+        "psk_last_capture": ([vp, vp, u64, ctypes.POINTER(u64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -209,6 +213,23 @@ class Context:
         rows = np.zeros(6 * 64, dtype=np.uint64)
         n = _lib.psk_last_levels(self._h, rows.ctypes.data, 64)
         return rows[: 6 * max(n, 0)].reshape(-1, 6)
+
+    def set_capture(self, level: int):
+        """Capture the translated reduced text of recursion level ``level`` in
+        the following calls (-1: off); read it with :meth:`last_capture`."""
+        self._check(_lib.psk_ctx_set_capture(self._h, int(level)))
+
+    def last_capture(self):
+        """The captured text of the last call (uint32 array), or None if the
+        last call did not name the captured level."""
+        cnt = ctypes.c_uint64(0)
+        rc = _lib.psk_last_capture(self._h, None, 0, ctypes.byref(cnt))
+        if rc == PSK_ENOTFOUND:
+            return None
+        self._check(rc)
+        out = np.empty(cnt.value, dtype=np.uint32)
+        self._check(_lib.psk_last_capture(self._h, out.ctypes.data, len(out), ctypes.byref(cnt)))
+        return out
 
     def last_timing_ms(self):
         ms = np.zeros(4, dtype=np.float64)
@@ -344,6 +365,18 @@ def check8_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
     ok = ctypes.c_int(0)
     rc = _lib.psk_check8_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
                              ctypes.byref(ok), _stream_ptr(stream))
+    _raise(rc, ctx)
+    return bool(ok.value)
+
+
+def check32_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
+                stream=None) -> bool:
+    """Device-side SA check of an integer (u32) text."""
+    if n is None:
+        n = d_text.numel()
+    ok = ctypes.c_int(0)
+    rc = _lib.psk_check32_dev(_ctx_handle(ctx), _dev_ptr(d_text), n, _dev_ptr(d_sa),
+                              ctypes.byref(ok), _stream_ptr(stream))
     _raise(rc, ctx)
     return bool(ok.value)
 

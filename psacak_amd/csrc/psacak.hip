@@ -148,6 +148,34 @@ void run32_dev(psk_ctx *c, const u32 *d_text, u64 n, u32 *d_sa, bool raw) {
     t.stop();
 }
 
+// The SA check of bin/psacak.rs:73-80 in linear time (Burkhardt and
+// Kaerkkaeinen): sa is a permutation of [0, n) (every slot claims its suffix
+// once), and for every adjacent pair (a, b) either T[a] < T[b], or T[a] == T[b]
+// and the suffix after a ranks below the one after b (the empty suffix
+// lowest).  bad: 1 = not a permutation, 2 = order violated.  rank: n u32.
+template <class C>
+void check_sa_dev(Dev &d, const C *text, u64 n, const u32 *sa, u32 *rank, u32 *bad) {
+    const u64 nn = n;
+    d.fill_u32(rank, 0xFFFFFFFFu, n);
+    for_each(d, n, [=] __device__(u64 i) {
+        u32 x = sa[i];
+        if (x >= nn) { atomicOr(bad, 1u); return; }
+        u32 old = atomicExch(&rank[x], (u32)i);
+        if (old != 0xFFFFFFFFu) atomicOr(bad, 1u);
+    });
+    for_each(d, n - 1, [=] __device__(u64 i) {
+        u32 a = sa[i], b = sa[i + 1];
+        if (a >= nn || b >= nn) return;
+        C ca = text[a], cb = text[b];
+        if (ca > cb) { atomicOr(bad, 2u); return; }
+        if (ca == cb) {
+            i64 ra = a + 1 < nn ? (i64)rank[a + 1] : -1;
+            i64 rb = b + 1 < nn ? (i64)rank[b + 1] : -1;
+            if (ra >= rb) atomicOr(bad, 2u);
+        }
+    });
+}
+
 hipStream_t pick_stream(psk_ctx *c, void *stream) {
     return stream ? (hipStream_t)stream : c->stream;
 }
@@ -420,6 +448,26 @@ int psk_last_levels(const psk_ctx *ctx, uint64_t *rows, int max_levels) {
     return n;
 }
 
+int psk_ctx_set_capture(psk_ctx *ctx, int level) {
+    if (!ctx || level < -1 || level > 63) return PSK_EINVAL;
+    ctx->rt.capture_depth = level;
+    ctx->rt.captured.clear();
+    ctx->rt.captured_ok = false;
+    return PSK_OK;
+}
+
+int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *count) {
+    if (!ctx || !count) return PSK_EINVAL;
+    if (!ctx->rt.captured_ok) return PSK_ENOTFOUND;
+    const u64 m = ctx->rt.captured.size();
+    *count = m;
+    if (out) {
+        if (cap < m) return PSK_EINVAL;
+        if (m) memcpy(out, ctx->rt.captured.data(), m * sizeof(u32));
+    }
+    return PSK_OK;
+}
+
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries) {
     if (!ctx || !ms) return PSK_EINVAL;
     int n = std::min(max_entries, 4);
@@ -556,30 +604,8 @@ int psk_check8_dev(psk_ctx *ctx, const uint8_t *d_text, uint64_t n, const uint32
             u32 *rank = d.a->get<u32>(n);
             u32 *bad = d.a->get<u32>(1);
             d.zero(bad, sizeof(u32));
-            const u64 nn = n;
-            d.fill_u32(rank, 0xFFFFFFFFu, n);
-            // permutation: every slot claims its suffix exactly once
-            for_each(d, n, [=] __device__(u64 i) {
-                u32 x = d_sa[i];
-                if (x >= nn) { atomicOr(bad, 1u); return; }
-                u32 old = atomicExch(&rank[x], (u32)i);
-                if (old != 0xFFFFFFFFu) atomicOr(bad, 1u);
-            });
-            // Burkhardt-Kaerkkaeinen adjacent-pair condition
-            for_each(d, n - 1, [=] __device__(u64 i) {
-                u32 a = d_sa[i], b = d_sa[i + 1];
-                if (a >= nn || b >= nn) return;
-                u8 ca = d_text[a], cb = d_text[b];
-                if (ca > cb) { atomicOr(bad, 2u); return; }
-                if (ca == cb) {
-                    i64 ra = a + 1 < nn ? (i64)rank[a + 1] : -1;
-                    i64 rb = b + 1 < nn ? (i64)rank[b + 1] : -1;
-                    if (ra >= rb) atomicOr(bad, 2u);
-                }
-            });
-            u32 b = d.read_u32(bad);
-            *ok = b == 0;
-            mk.reset();
+            check_sa_dev(d, d_text, n, d_sa, rank, bad);
+            *ok = d.read_u32(bad) == 0;
         });
     });
 }
@@ -608,6 +634,28 @@ int psk_psacak8_dev_noalloc(const uint8_t *d_text, uint64_t n, uint32_t *d_sa,
         psacak8_run(g.rt, d_text, n, d_sa);
         const u32 e = d.read_u32(d.err);
         if (e) fail(PSK_EDEVICE, "internal index check failed (flags " + std::to_string(e) + ")");
+    });
+}
+
+int psk_check32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, const uint32_t *d_sa,
+                    int *ok, void *stream) {
+    if (!ok) return PSK_EINVAL;
+    *ok = 0;
+    int rc = check32(d_text, n, d_sa, n);
+    if (rc != PSK_OK) return rc;
+    if (n == 0) { *ok = 1; return PSK_OK; }
+    psk_ctx *c = ctx ? ctx : default_ctx();
+    return guarded(c, [&] {
+        ensure_ready(c);
+        on_stream(c, stream, [&] {
+            Dev &d = c->rt.d;
+            ArenaScope mk(*d.a);
+            u32 *rank = d.a->get<u32>(n);
+            u32 *bad = d.a->get<u32>(1);
+            d.zero(bad, sizeof(u32));
+            check_sa_dev(d, d_text, n, d_sa, rank, bad);
+            *ok = d.read_u32(bad) == 0;
+        });
     });
 }
 

@@ -46,6 +46,11 @@ struct Runtime {
     u32 doubling_distinct = 128;
     bool doubling_gave_up = false;  // in this call (reset by each entry point)
     int depth = 0;                  // recursion level being solved (traces)
+    // capture_depth >= 0: the translated reduced text of that level (the
+    // next level's text) is copied to the host as the level computes it
+    int capture_depth = -1;
+    std::vector<u32> captured;
+    bool captured_ok = false;
     // round 1 of direct naming at level 0 through a dictionary of the
     // distinct keys (false: by sorting every key)
     bool dict_naming = true;
@@ -314,14 +319,25 @@ inline void recurse_reduced(Runtime &rt, const u32 *reduced, const u32 *tails, u
                             u32 *SA1, int depth) {
     Dev &d = rt.d;
     ArenaScope mk(*d.a);
+    auto capture = [&](const u32 *text1) {
+        if (rt.capture_depth != depth) return;
+        rt.captured.resize(m1);
+        PSK_HIP(hipMemcpyAsync(rt.captured.data(), text1, (u64)m1 * sizeof(u32),
+                               hipMemcpyDeviceToHost, d.s));
+        PSK_HIP(hipStreamSynchronize(d.s));
+        rt.captured_ok = true;
+    };
     if (k < m1) {
         u32 *sb = d.a->get<u32>(((u64)m1 + 31) / 32 + 1);
         compute_types(d, reduced, m1, sb);
         u32 *s1 = d.a->get<u32>(m1);
         translate_names(d, reduced, s1, m1, sb, tails);
+        capture(s1);
         // the names stay as the dense symbol ranks of the next level
         level32(rt, s1, m1, sb, SA1, depth + 1, reduced, tails, k);
     } else {
+        // every name distinct: the bucket-pointer form is the names themselves
+        capture(reduced);
         inverse_names(d, reduced, m1, SA1);
     }
     mk.reset();

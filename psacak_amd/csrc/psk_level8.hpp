@@ -99,6 +99,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     rt.levels.clear();
     rt.max_rounds = 0;
     rt.doubling_gave_up = false;
+    rt.captured_ok = false;
     if (n == 0) return;
     const u32 *EB = bi ? bi->endbits : nullptr;
     const u32 *SB = bi ? bi->startbits : nullptr;
@@ -362,6 +363,7 @@ inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
     rt.levels.clear();
     rt.max_rounds = 0;
     rt.doubling_gave_up = false;
+    rt.captured_ok = false;
     if (m == 0) return;
     ArenaScope mk(*d.a);
     u32 *sb = d.a->get<u32>((m + 31) / 32 + 1);

@@ -196,3 +196,15 @@ def test_default_context_trim_and_workspace():
     assert P._lib.psk_ctx_trim(None) == P.PSK_OK
     assert P._lib.psk_ctx_workspace(None, ctypes.byref(r), None) == P.PSK_OK
     assert r.value == 0
+
+
+def test_capture_settings_without_gpu():
+    with P.Context() as c:
+        c.set_capture(0)
+        c.set_capture(-1)
+        assert c.last_capture() is None
+        for v in (-2, 64):
+            with pytest.raises(P.PskError) as e:
+                c.set_capture(v)
+            assert e.value.code == P.PSK_EINVAL
+    assert P._lib.psk_check32_dev(None, None, 4, None, None, None) == P.PSK_EINVAL

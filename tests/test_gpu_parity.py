@@ -537,3 +537,120 @@ def test_failed_call_leaves_workspace_reusable():
         z = gen.zipf32(1 << 16, sigma=1 << 12, seed=4)
         assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
         assert c.workspace()[0] <= budget
+
+
+def test_c3_zipf_sigma_2_20_against_oracle(ctx):
+    # the psacak32 configuration's own alphabet: Zipf(1.1) over 2^20 symbols
+    # with 20% exact repeats of 16-4096 symbols (psacak32.rs:912-943
+    # translation, then psacak32)
+    note("C3 zipf32 n=2^22 sigma=2^20")
+    z = gen.zipf32(1 << 22, sigma=1 << 20, s=1.1, repeat_frac=0.2, seed=20260103)
+    got = P.psacak32_raw(z, ctx)
+    assert np.array_equal(got, oracle.sacak32_raw(z))
+
+
+def test_check32_dev_full_c3_text(ctx):
+    # the full 2^28-symbol C3 text, verified on the device (permutation plus
+    # the adjacent-pair rank condition), and planted mistakes caught
+    import torch
+    note("C3 zipf32 n=2^28 device check")
+    n = 1 << 28
+    z = gen.zipf32(n, sigma=1 << 20, s=1.1, repeat_frac=0.2, seed=20260103)
+    dz = torch.from_numpy(z.view(np.int32)).to("cuda:0")
+    del z
+    ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    P.psacak32_raw_dev(dz, ds, n, ctx=ctx)
+    torch.cuda.synchronize()
+    assert P.check32_dev(dz, ds, n, ctx=ctx)
+    a, b = ds[12345].item(), ds[12346].item()
+    ds[12345], ds[12346] = b, a
+    assert not P.check32_dev(dz, ds, n, ctx=ctx)
+    ds[12345], ds[12346] = a, b
+    ds[7] = ds[8].item()
+    assert not P.check32_dev(dz, ds, n, ctx=ctx)
+    del dz, ds
+    torch.cuda.empty_cache()
+
+
+def test_check32_dev_small_against_oracle_checker(ctx):
+    import torch
+    note("check32_dev small")
+    rng = np.random.default_rng(3)
+    for it in range(20):
+        n = int(rng.integers(2, 3000))
+        u = rng.integers(0, int(rng.choice([2, 5, 100, 1 << 20])), n).astype(np.uint32)
+        sa = oracle.sacak32_raw(u)
+        du = torch.from_numpy(u.view(np.int32)).to("cuda:0")
+        ds = torch.from_numpy(sa.view(np.int32).copy()).to("cuda:0")
+        assert P.check32_dev(du, ds, n, ctx=ctx)
+        bad = sa.copy()
+        i = int(rng.integers(0, n - 1))
+        bad[i], bad[i + 1] = bad[i + 1], bad[i]
+        ds = torch.from_numpy(bad.view(np.int32)).to("cuda:0")
+        assert P.check32_dev(du, ds, n, ctx=ctx) == (oracle.check_sa32(u, bad) == 0)
+
+
+def _oracle_reduced(text_level, width):
+    """The oracle's translated reduced text of one level (stage 1, names,
+    translation): the next level's text, and its number of names."""
+    if width == 1:
+        lms = oracle.stage1_8(text_level)
+        k, red = oracle.name8(text_level, lms)
+    else:
+        lms = oracle.stage1_32(text_level)
+        k, red = oracle.name32(text_level, lms)
+    return k, red
+
+
+@pytest.mark.parametrize("kind", ["markov", "iid", "uniform3", "runs"])
+def test_level_intermediates_match_oracle(kind):
+    # canonical intermediates of levels 0 and 1 (bit-exact): the per-level
+    # counters (m, m1, mL, k, sum of LMS-substring lengths), which also feed
+    # the bench's roofline model, and the translated reduced texts
+    # (naming.rs:20-274) the recursion receives
+    note(f"level intermediates {kind}")
+    if kind == "markov":
+        t = gen.markov_dna(1 << 22, seed=71)
+    elif kind == "iid":
+        t = gen.iid_dna(1 << 22, seed=72)
+    elif kind == "uniform3":
+        t = gen.uniform8(1 << 21, alphabet=3, seed=73)
+    else:
+        t = np.frombuffer(ADVERSARIAL["runs"] * 8, dtype=np.uint8)
+    ref_rows = oracle.level_stats8(t)
+    k0, red0 = _oracle_reduced(t, 1)
+    with P.Context() as c:
+        for level in (0, 1):
+            c.set_capture(level)
+            sa = P.psacak(t, c)
+            rows = c.last_levels()
+            cap = c.last_capture()
+            assert np.array_equal(rows[0, :5], ref_rows[0]), (rows[0], ref_rows[0])
+            assert rows[0, 3] == k0
+            if level == 0:
+                assert cap is not None and np.array_equal(cap, red0)
+            elif kind in ("markov", "iid") or (rows.shape[0] > 1 and rows[1, 1] != 0):
+                # level 1 was named (not sorted by prefix doubling)
+                assert np.array_equal(rows[1, :5], ref_rows[1]), (rows[1], ref_rows[1])
+                k1, red1 = _oracle_reduced(red0, 4)
+                assert rows[1, 3] == k1
+                assert cap is not None and np.array_equal(cap, red1)
+            elif rows.shape[0] > 1:
+                assert rows[1, 0] == ref_rows[1, 0]
+                assert cap is None
+        assert np.array_equal(sa, oracle.sacak8(t))
+
+
+def test_level_intermediates_psacak32_zipf():
+    note("level intermediates zipf32")
+    z = gen.zipf32(1 << 20, sigma=1 << 16, s=1.1, repeat_frac=0.2, seed=74)
+    tr = oracle.translate32(z)
+    k0, red0 = _oracle_reduced(tr, 4)
+    with P.Context() as c:
+        c.set_capture(0)
+        sa = P.psacak32_raw(z, c)
+        cap = c.last_capture()
+        rows = c.last_levels()
+    assert rows[0, 3] == k0
+    assert cap is not None and np.array_equal(cap, red0)
+    assert np.array_equal(sa, oracle.sacak32(tr))
