@@ -314,6 +314,136 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
     mk.reset();
 }
 
+// The same queue fill without host synchronisation, for texts whose runs of
+// equal bytes are all at most 256 long (so every depth fits one 8-bit
+// multi-split digit): the seed count a = *d_start stays on the device, and
+// every launch takes the host-known maximum shape (limit items; the items
+// past a, or past the limit - a chain entries, take no part).  d_tmp: 2 u32.
+inline void chain_queue8_dev(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *dig, QPos qp,
+                             u32 c, const u32 *d_start, u64 limit, u32 *d_tmp, u32 none) {
+    if (limit == 0) return;
+    u32 *err = d.err;
+    ArenaScope mk(*d.a);
+    u32 *run = d.a->get<u32>(limit + 1);
+    unsigned short *stop = d.a->get<unsigned short>(limit + 1);  // byte | has << 8
+    const u64 lim = limit;
+    {
+        int prec = d.pbegin("chain8_runs", 22.0 * limit);
+        for_each(d, lim, [=] __device__(u64 k) {
+            const u64 a = *d_start;
+            if (k >= a) return;
+            u32 r = 0;
+            const u64 slot = qp(k);
+            u32 x = SA[slot];
+            u64 w = PW[slot];
+            if (!Win::known(w)) {
+                w = pr.window_any(x, aligned);
+                PW[slot] = w;
+            }
+            const u32 v = Win::v(w);
+            const u32 cp = v ? Win::b(w, 1) : none;
+            dig[k] = (u8)cp;
+            u32 sb = 0;
+            if (v && cp == c) {
+                r = 1;
+                while (r < v && Win::b(w, r + 1) == c) r++;
+                if (r < v) {
+                    sb = Win::b(w, r + 1) | 256u;
+                } else if (!Win::e(w)) {
+                    r = pr.run_before(x);
+                    if (pr.has(x - r)) sb = pr.byte(x - r) | 256u;
+                }
+            }
+            run[k] = r;
+            stop[k] = (unsigned short)sb;
+        });
+        d.pend(prec);
+    }
+    // emission offsets of the seeds' chains; their total must fill the
+    // queue: a + total == limit
+    u32 *eo = d.a->get<u32>(limit + 1);
+    u32 *tot = d_tmp + 1;
+    scan_exclusive(
+        d, lim, [=] __device__(u64 k) { return k < *d_start ? run[k] : 0u; },
+        [=] __device__(u64 k, u32 ex, u32) { eo[k] = ex; }, tot);
+    u32 *ek = d.a->get<u32>(limit), *ev = d.a->get<u32>(limit);
+    u8 *eb = d.a->get<u8>(limit);
+    u64 *ew = d.a->get<u64>(limit);
+    for_each(d, lim, [=] __device__(u64 k) {
+        const u64 a = *d_start;
+        if (k == 0 && (a > lim || (u64)*tot + a != lim)) atomicOr(err, 1024u);
+        if (k >= a) return;
+        u32 r = run[k];
+        if (!r) return;
+        const u64 slot = qp(k);
+        u32 x = SA[slot], o = eo[k];
+        if ((u64)o + r > lim) return;  // flagged above
+        u64 w = PW[slot];
+        const u32 st = stop[k];
+        const bool hs = st >> 8;
+        const u32 sb = st & 255u;
+        for (u32 t = 1; t <= r; t++) {
+            ek[o + t - 1] = t - 1;
+            ev[o + t - 1] = x - t;
+            eb[o + t - 1] = t < r ? (u8)c : (u8)(hs ? sb : none);
+            ew[o + t - 1] = chain_window(w, r, c, hs, sb, t);
+        }
+    });
+    // breadth-first order: a stable multi-split of the emitted entries by depth
+    multisplit(
+        d, lim,
+        [=] __device__(u64 j, u32 *key) {
+            if (j >= lim - *d_start) return false;
+            *key = ek[j];
+            return true;
+        },
+        [=] __device__(u64 j, u32, u32 gpos, u32) {
+            const u64 q = *d_start + gpos;
+            if (q < lim) {
+                const u64 slot = qp(q);
+                SA[slot] = ev[j];
+                PW[slot] = ew[j];
+                dig[q] = eb[j];
+            } else {
+                atomicOr(err, 2u);
+            }
+        },
+        nullptr);
+}
+
+// Longest run of equal bytes in the text is at most 256?  From the decided
+// bits DB (bit p: T[p] != T[p+1] or p ends a text): a run ends at each
+// decided position, and starts after the previous one.
+inline bool runs_at_most_256(Dev &d, const u32 *DB, u64 n, u32 *d_flag) {
+    const u64 nwords = (n + 31) / 32;
+    d.zero(d_flag, sizeof(u32));
+    for_each(d, nwords, [=] __device__(u64 w) {
+        const u32 x = DB[w];
+        if (!x) return;
+        // the run ending at the lowest decided position of this word starts
+        // after the previous decided position (or at the text start)
+        const u64 p = w * 32 + (u64)(__ffs(x) - 1);
+        u64 q = w;
+        bool found = false;
+        u64 prev = 0;
+        for (int back = 0; back < 10 && q > 0; back++) {
+            const u32 y = DB[--q];
+            if (y) {
+                prev = q * 32 + (u64)(31 - __clz(y));
+                found = true;
+                break;
+            }
+        }
+        if (!found && q > 0) {
+            atomicOr(d_flag, 1u);  // more than 9 words without a decided bit
+            return;
+        }
+        const u64 len = found ? p - prev : p + 1;
+        if (len > 256) atomicOr(d_flag, 1u);
+    });
+    return d.read_u32(d_flag) == 0;
+}
+
 // Per-bucket device tables: head, tail, fill (induce counters).
 struct Induce8Tables {
     u32 *head, *tail, *fill, *counts, *tmp;
@@ -330,6 +460,23 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
     const Pred pr{T, SB, DB};
     const bool aligned = (((uintptr_t)T) & 7u) == 0;
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
+    // Small queues (many buckets: large alphabets) are filled without host
+    // synchronisation unless a run of equal bytes exceeds 256 (chain depths
+    // past one multi-split digit); large ones take the exact sizes, whose
+    // readback costs less than the maximum-shape launches would
+    constexpr u64 SYNC_FREE_MAX = 1ull << 22;
+    bool any_small = false;
+    for (int c = 0; c < 256; c++) {
+        const u64 nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
+        if ((nl && nl < SYNC_FREE_MAX) || (ns && ns < SYNC_FREE_MAX)) any_small = true;
+    }
+    const bool short_runs = any_small && runs_at_most_256(d, DB, n, tb.tmp);
+    auto queue = [&](QPos qp, u32 c, const u32 *d_start, u64 limit, u32 none) {
+        if (short_runs && limit < SYNC_FREE_MAX)
+            chain_queue8_dev(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);
+        else
+            chain_queue8(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);
+    };
     u32 *err = d.err;
     const u64 nn = n;
     // ---------------- L-scan ----------------
@@ -361,7 +508,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], nlms = B.nLMS[c];
         const u32 cc = (u32)c;
-        if (nl) chain_queue8(d, pr, aligned, SA, PW, dig, QPos{hc, 1}, cc, fill + c, nl, tb.tmp, 0u);
+        if (nl) queue(QPos{hc, 1}, cc, fill + c, nl, 0u);
         const u64 R = nl + nlms;
         if (c == 255 || R == 0) continue;  // nothing can be induced above 255
         auto pos = [=] __device__(u64 j) -> u64 { return j < nl ? hc + j : tc - nlms + (j - nl); };
@@ -407,9 +554,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
         if (!B.cnt[c]) continue;
         const u64 hc = B.head[c], tc = B.head[c + 1], nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
         const u32 cc = (u32)c;
-        if (ns)
-            chain_queue8(d, pr, aligned, SA, PW, dig, QPos{tc - 1, -1}, cc, fill + c, ns, tb.tmp,
-                         255u);
+        if (ns) queue(QPos{tc - 1, -1}, cc, fill + c, ns, 255u);
         const u64 R = ns + nl;
         if (c == 0 || R == 0) continue;  // nothing can be induced below 0
         auto pos = [=] __device__(u64 j) -> u64 { return j < ns ? tc - 1 - j : hc + nl - 1 - (j - ns); };

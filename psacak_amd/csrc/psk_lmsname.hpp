@@ -491,7 +491,8 @@ __device__ __forceinline__ u32 dict_hash(u64 key, int bits) {
 // (each in one of two places), so a frequent key goes to the table once per
 // workgroup and its slot is its hash: a wave waits for the table whenever
 // one of its lanes misses, so the cache holds a whole byte text's
-// dictionary.  ctr[0] counts distinct keys, ctr[1] flags a full table.
+// dictionary.  ctr[0] counts distinct keys, ctr[1] flags a full table or
+// more than DICT_MAX_KEYS distinct keys (the insert then stops early).
 __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, const u32 *lms_text,
                                                             u32 m1, u64 m, int D, int Db, u64 term,
                                                             LmsEnds E, u64 *tkey, u32 *ctr,
@@ -500,6 +501,7 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
     for (int i = threadIdx.x; i < DICT_LDS; i += DICT_BLOCK) seen[i] = DICT_EMPTY;
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
+    bool abandon = false;
     for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
          r += (u64)gridDim.x * DICT_BLOCK) {
         u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
@@ -518,10 +520,17 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
             u64 cur = tkey[s];
             if (cur == key) { ok = true; break; }
             if (cur == DICT_EMPTY) {
+                // a dictionary past its capacity is abandoned (the caller
+                // sorts instead): the lane leaves (no further inserts, no
+                // flag atomics), so the table stays about half full at most
+                if (__hip_atomic_load(&ctr[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    abandon = true;
+                    break;
+                }
                 u64 old = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)DICT_EMPTY,
                                     (unsigned long long)key);
                 if (old == DICT_EMPTY) {
-                    atomicAdd(&ctr[0], 1u);
+                    if (atomicAdd(&ctr[0], 1u) >= DICT_MAX_KEYS) atomicOr(&ctr[1], 4u);
                     ok = true;
                     break;
                 }
@@ -529,6 +538,7 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
             }
             s = (s + 1) & mask;
         }
+        if (abandon) break;
         if (ok) {
             SLOT[r] = s;
             if (s == home) seen[seen[c1] == DICT_EMPTY ? c1 : c2] = key;
