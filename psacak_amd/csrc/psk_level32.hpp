@@ -138,6 +138,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     }
     u64 h = 1;
     int rounds = 0;
+    u32 G = 0;             // tie groups entering the round (after round 1)
     bool carried = false;  // the group ids hold run offsets
     bool skipped = false;  // a round left labels unwritten
     while (A > 0) {
@@ -175,7 +176,11 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        << RUN_KEY_SHIFT;
             keys[i] = key;
         });
-        bool sw = radix_sort_kv(d, A, keys, act, kt, vt, Bh + B);
+        // round 1 sorts everything; later rounds keep every tie group in its
+        // place and sort it by the label bits alone
+        bool sw = first ? radix_sort_kv(d, A, keys, act, kt, vt, Bh + B)
+                        : group_sort_kv(d, A, G, gid, prevc ? RUN_GID_MASK : ~0u, keys, act, kt,
+                                        vt, B);
         const u64 *K = sw ? kt : keys;
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
@@ -220,6 +225,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             trace_round(d, what, rounds, AA, Bh + B, gid, ag[0], cr ? RUN_GID_MASK : ~0u, ag[1]);
         }
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
+        G = ag[1];
         carried = cr;
         skipped = skipped || cr;
         h *= 2;

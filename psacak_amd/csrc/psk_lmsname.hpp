@@ -316,6 +316,8 @@ struct LmsRounds {
     u32 A = 0;
     u64 off = 0;  // digits compared so far by every tied substring
     int Bg = 0;   // bits of the group ids
+    u32 G = 0;    // tie groups (dense ids 0..G-1 in gid)
+    bool grouped = false;  // act holds the tie groups contiguously, in id order
     int rounds = 0;
 
     void alloc(Dev &d, u32 m1) {
@@ -359,7 +361,12 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
             u64 hi = first ? 0ull : (u64)gi[i];
             keys[i] = (hi << LB) | lms_key(rk, lms_text, mm1, mm, r, o, DD, DB, term, E);
         });
-        bool sw = radix_sort_kv(d, st.A, st.keys, st.act, st.kt, st.vt, st.Bg + LB);
+        // after round 1 the tie groups stay in place: each is sorted by its
+        // new digits alone
+        bool sw = !st.grouped
+                      ? radix_sort_kv(d, st.A, st.keys, st.act, st.kt, st.vt, st.Bg + LB)
+                      : group_sort_kv(d, st.A, st.G, st.gid, ~0u, st.keys, st.act, st.kt, st.vt,
+                                      LB);
         const u64 *K = sw ? st.kt : st.keys;
         const u32 *V = sw ? st.vt : st.act;
         u32 *nxt = sw ? st.act : st.vt;
@@ -391,6 +398,8 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
         std::swap(st.glab, st.glab2);
         trace_round(d, "lms names", st.rounds, before, st.Bg + LB, st.gid, st.A, ~0u, ag[1]);
         st.Bg = bit_width(ag[1] ? ag[1] - 1 : 0);
+        st.G = ag[1];
+        st.grouped = true;  // the relabel emits the kept items by group
         st.off += (u64)D;
     }
     return true;
@@ -747,6 +756,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
             ctr + 3);
         st.A = d.read_u32(ctr + 3);
         st.Bg = bit_width(G - 1);
+        st.G = G;
         E.flag = ctr + 3;  // free again after the count of tied substrings
         d.zero(E.flag, sizeof(u32));
         if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||

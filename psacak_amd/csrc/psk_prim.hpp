@@ -788,6 +788,296 @@ inline int bit_width(u64 x) {
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Group sort.  Items [0, A) come in G contiguous groups (group g holds items
+// [start[g], start[g+1]), groups in order), as the tie groups of a
+// refinement round do.  Every group is sorted stably by the key bits
+// [0, lob) and stays where it is, so the result equals a stable sort of the
+// whole array by (group, low bits) -- without sorting on the group bits:
+//  * groups of at most GS_LOCAL items are ranked by counting inside one
+//    workgroup (each item counts the members that sort before it);
+//  * larger groups go through stable LSD passes over the low bits alone,
+//    with tiles that never cross a group boundary and a count matrix laid
+//    out group by group ([group][digit][tile of the group]), so one global
+//    scan gives every (digit, tile) its position inside its group.
+// ---------------------------------------------------------------------------
+constexpr u32 GS_LOCAL = 256;  // largest group ranked by counting
+constexpr u32 GS_WIN = 1024;   // group starts per ranking workgroup
+
+// Tiles of the large groups: tile t belongs to group tg[t], is that group's
+// tile t - tb[g], and covers its items [start[g] + RS_TILE (t - tb[g]), ...).
+struct GroupTiles {
+    const u32 *start, *tb, *tg, *bias;
+    __device__ __forceinline__ void tile(u32 t, u64 &base, u32 &cnt, u64 &mbase, u32 &mstride,
+                                         u64 &obias) const {
+        const u32 g = tg[t];
+        const u32 j = t - tb[g];
+        const u32 s0 = start[g], s1 = start[g + 1];
+        const u32 nt = tb[g + 1] - tb[g];
+        base = (u64)s0 + (u64)j * RS_TILE;
+        const u64 rem = (u64)s1 - base;
+        cnt = (u32)(rem < RS_TILE ? rem : RS_TILE);
+        mbase = (u64)tb[g] * 256 + j;
+        mstride = nt;
+        obias = bias[g];
+    }
+};
+
+// Ranking of the small groups whose first item lies in this workgroup's
+// window; their items move to start[g] + rank (in place or into ko/vo).
+__global__ void __launch_bounds__(BLOCK) k_gs_local(const u64 *K, const u32 *V, u64 *ko, u32 *vo,
+                                                    u64 A, const u32 *gid, u32 gmask,
+                                                    const u32 *start, u64 lomask) {
+    constexpr u32 CAP = GS_WIN + GS_LOCAL;
+    __shared__ u64 sk[CAP];
+    __shared__ u32 sv[CAP];
+    __shared__ unsigned short sg0[CAP], sg1[CAP];  // the item's group, local bounds
+    const u64 w0 = (u64)blockIdx.x * GS_WIN;
+    if (w0 >= A) return;
+    const u64 w1 = w0 + GS_WIN < A ? w0 + GS_WIN : A;
+    const u32 ga = gid[w0] & gmask;
+    const u32 gfirst = start[ga] == w0 ? ga : ga + 1;
+    const u32 glast = gid[w1 - 1] & gmask;
+    if (gfirst > glast) return;  // one group covers the whole window
+    const u64 lo = start[gfirst];
+    u64 hi = start[glast + 1];
+    // the last group may run past the window: a small one is ranked here
+    // (at most GS_LOCAL - 1 items past it), a large one elsewhere
+    if (hi - start[glast] > GS_LOCAL) hi = start[glast];
+    const u32 nl = (u32)(hi - lo);
+    for (u32 i = threadIdx.x; i < nl; i += BLOCK) {
+        const u64 p = lo + i;
+        const u32 g = gid[p] & gmask;
+        const u32 a = start[g], b = start[g + 1];
+        if (b - a <= GS_LOCAL) {
+            sk[i] = K[p];
+            sv[i] = V[p];
+            sg0[i] = (unsigned short)(a - lo);
+            sg1[i] = (unsigned short)(b - lo);
+        } else {
+            sg0[i] = sg1[i] = 0xFFFF;  // a large group's item: not ranked here
+        }
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nl; i += BLOCK) {
+        const u32 a = sg0[i];
+        if (a == 0xFFFF) continue;
+        const u32 b = sg1[i];
+        const u64 key = sk[i], mine = key & lomask;
+        u32 r = 0;
+        for (u32 j = a; j < b; j++) {
+            const u64 x = sk[j] & lomask;
+            r += (x < mine) || (x == mine && j < i);
+        }
+        ko[lo + a + r] = key;
+        vo[lo + a + r] = sv[i];
+    }
+}
+
+// Digit counts of one group tile per workgroup: hist[mbase + d * mstride].
+template <class Tiles>
+__global__ void __launch_bounds__(BLOCK) k_gs_count(const u64 *__restrict__ ki, int shift,
+                                                    u32 *hist, u32 nt, Tiles tiles) {
+    __shared__ u32 h[256];
+    h[threadIdx.x] = 0;
+    const u32 t = xcd_tile(blockIdx.x, nt);
+    u64 base, mbase, ob;
+    u32 cnt, ms;
+    tiles.tile(t, base, cnt, mbase, ms, ob);
+    __syncthreads();
+    for (u32 r = 0; r < RS_ROUNDS; r++) {
+        const u32 j = r * BLOCK + threadIdx.x;
+        const u32 x = j < cnt ? (u32)(ki[base + j] >> shift) & 255u : 256u;
+        const u32 x0 = __shfl(x, 0, WAVE);
+        if (ballot(x == x0) == ~0ull) {
+            if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[x0], (u32)WAVE);
+        } else if (x < 256u) {
+            atomicAdd(&h[x], 1u);
+        }
+    }
+    __syncthreads();
+    hist[mbase + (u64)threadIdx.x * ms] = h[threadIdx.x];
+}
+
+// Stable scatter of one group tile (the ranking of k_rs_scatter): off holds
+// the exclusive scan of the group-ordered count matrix (plus the total), so
+// the count of (digit, tile) is the difference to the next entry.
+template <class Tiles>
+__global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki,
+                                                      const u32 *__restrict__ vi,
+                                                      u64 *__restrict__ ko, u32 *__restrict__ vo,
+                                                      int shift, const u32 *off, u32 nt,
+                                                      Tiles tiles) {
+    __shared__ u64 sk[RS_TILE];
+    __shared__ u32 sv[RS_TILE];
+    __shared__ u32 tstart[256];
+    __shared__ u64 gbase[256];
+    __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
+    __shared__ u32 red[NWAVES + 1];
+    const u32 t = xcd_tile(blockIdx.x, nt);
+    u64 base, mbase, ob;
+    u32 cnt, ms;
+    tiles.tile(t, base, cnt, mbase, ms, ob);
+    const u32 dg = threadIdx.x;
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++)
+        #pragma unroll
+        for (int w = 0; w < NWAVES; w++) wcnt[r][w][dg] = 0;
+    {
+        // a group's entries are tb*256 + d*nt_g + j: the entry after (d, j)
+        // is (d, j+1), or (d+1, 0), or the next group's first entry (the
+        // appended total after the last group) -- always the next index
+        const u64 o = mbase + (u64)dg * ms;
+        const u32 g = off[o];
+        const u32 cn = off[o + 1] - g;
+        u32 tot;
+        const u32 ex = block_exclusive_sum(cn, red, &tot);  // includes a barrier
+        tstart[dg] = ex;
+        gbase[dg] = (u64)g + ob;
+    }
+    const int w = wave_id();
+    u64 key[RS_ROUNDS];
+    u32 val[RS_ROUNDS];
+    u32 dk[RS_ROUNDS];
+    unsigned short rk[RS_ROUNDS];
+    bool vv[RS_ROUNDS];
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const u32 j = (u32)r * BLOCK + threadIdx.x;
+        vv[r] = j < cnt;
+        key[r] = vv[r] ? ki[base + j] : 0ull;
+        val[r] = vv[r] ? vi[base + j] : 0u;
+    }
+    __syncthreads();  // wcnt zeroed
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        dk[r] = (u32)(key[r] >> shift) & 255u;
+        u64 peers = match_key8(vv[r], dk[r]);
+        rk[r] = (unsigned short)__popcll(peers & lanemask_lt());
+        if (vv[r] && rk[r] == 0) wcnt[r][w][dk[r]] = (unsigned short)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        u32 run = tstart[dg];
+        #pragma unroll
+        for (int r = 0; r < RS_ROUNDS; r++)
+            #pragma unroll
+            for (int w2 = 0; w2 < NWAVES; w2++) {
+                u32 c = wcnt[r][w2][dg];
+                wcnt[r][w2][dg] = (unsigned short)run;
+                run += c;
+            }
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        if (vv[r]) {
+            u32 lp = (u32)wcnt[r][w][dk[r]] + rk[r];
+            sk[lp] = key[r];
+            sv[lp] = val[r];
+        }
+    }
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < cnt; j += BLOCK) {
+        u64 k = sk[j];
+        u32 d = (u32)(k >> shift) & 255u;
+        u64 g = gbase[d] + (j - tstart[d]);
+        ko[g] = k;
+        vo[g] = sv[j];
+    }
+}
+
+// Group sort of (k, v)[0..A) with G groups whose ids are gid[i] & gmask
+// (dense, non-decreasing, 0 <= id < G): each group sorted stably by the key
+// bits [0, lob).  Returns true if the result is in (kt, vt).
+inline bool group_sort_kv(Dev &d, u64 A, u32 G, const u32 *gid, u32 gmask, u64 *k, u32 *v,
+                          u64 *kt, u32 *vt, int lob) {
+    if (A == 0 || G == 0) return false;
+    ArenaScope mk(*d.a);
+    u32 *start = d.a->get<u32>((u64)G + 1);
+    u32 *tb = d.a->get<u32>((u64)G + 1);
+    u32 *bias = d.a->get<u32>((u64)G + 1);
+    u32 *ctr = d.a->get<u32>(2);
+    const u64 AA = A;
+    const u32 GG = G;
+    for_each(d, AA, [=] __device__(u64 i) {
+        const u32 g = gid[i] & gmask;
+        if (i == 0 || (gid[i - 1] & gmask) != g) start[g] = (u32)i;
+        if (i + 1 == AA) start[GG] = (u32)AA;
+    });
+    // tiles of the large groups, and each large group's shift from its place
+    // among the large groups' items to its place in the array
+    scan_exclusive(
+        d, GG,
+        [=] __device__(u64 g) {
+            const u32 s = start[g + 1] - start[g];
+            return s > GS_LOCAL ? (u32)((s + RS_TILE - 1) / RS_TILE) : 0u;
+        },
+        [=] __device__(u64 g, u32 ex, u32 v) {
+            tb[g] = ex;
+            if (g + 1 == GG) tb[GG] = ex + v;
+        },
+        ctr);
+    scan_exclusive(
+        d, GG,
+        [=] __device__(u64 g) {
+            const u32 s = start[g + 1] - start[g];
+            return s > GS_LOCAL ? s : 0u;
+        },
+        [=] __device__(u64 g, u32 ex, u32) { bias[g] = start[g] - ex; }, ctr + 1);
+    u32 hc[2];
+    d.read(hc, ctr, sizeof(hc));
+    const u32 NT = hc[0];
+    const int passes = NT ? (lob + 7) / 8 : 0;
+    const bool swapped = (passes & 1) != 0;
+    const u64 lomask = lob >= 64 ? ~0ull : ((1ull << lob) - 1);
+    {
+        int pr = d.pbegin("k_gs_local", 32.0 * (double)(A - hc[1]));
+        k_gs_local<<<ceil_div(A, GS_WIN), BLOCK, 0, d.s>>>(k, v, swapped ? kt : k, swapped ? vt : v,
+                                                          A, gid, gmask, start, lomask);
+        PSK_LAUNCH_CHECK();
+        d.pend(pr);
+    }
+    if (NT) {
+        u32 *tg = d.a->get<u32>(NT);
+        for_each(d, NT, [=] __device__(u64 t) {
+            // last group whose first tile is at or before t
+            u32 lo = 0, hi = GG;
+            while (hi - lo > 1) {
+                const u32 mid = (lo + hi) >> 1;
+                if (tb[mid] <= (u32)t) lo = mid;
+                else hi = mid;
+            }
+            tg[t] = lo;
+        });
+        const GroupTiles tiles{start, tb, tg, bias};
+        const u64 total = (u64)256 * NT;
+        u32 *hist = d.a->get<u32>(total + 1);
+        const double items = (double)hc[1];
+        bool sw = false;
+        for (int p = 0; p < passes; p++) {
+            const int shift = 8 * p;
+            const u64 *ki = sw ? kt : k;
+            const u32 *vi = sw ? vt : v;
+            u64 *ko = sw ? k : kt;
+            u32 *vo = sw ? v : vt;
+            int pc = d.pbegin("k_gs_count", 8.0 * items);
+            k_gs_count<<<NT, BLOCK, 0, d.s>>>(ki, shift, hist, NT, tiles);
+            PSK_LAUNCH_CHECK();
+            d.pend(pc);
+            scan_exclusive(
+                d, total, [=] __device__(u64 i) { return hist[i]; },
+                [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, hist + total);
+            int ps = d.pbegin("k_gs_scatter", 24.0 * items);
+            k_gs_scatter<<<NT, BLOCK, 0, d.s>>>(ki, vi, ko, vo, shift, hist, NT, tiles);
+            PSK_LAUNCH_CHECK();
+            d.pend(ps);
+            sw = !sw;
+        }
+    }
+    return swapped;
+}
+
 // Trace line of a refinement round: items and groups before and after, and
 // the items left tied by the size of their group (log2 classes).  gid: group
 // id of every kept item (masked by gmask), contiguous groups.
