@@ -108,10 +108,21 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
 // sorts, items = every position of one type): labels that no later round and
 // no later sort reads are not written (ChainSink), using the items' run
 // offsets carried in their keys while the key bits leave room.
+// final (with rank, runbits and SA; 1 = the final L sort, 2 = the final S
+// sort): items come as [the items whose successor's slot is already final,
+// in that slot's order] ++ [the others, in any order].  Round 1 then sorts by
+// (rank(p), rank(p+1), type of p+1) only: a stable sort keeps the first part
+// in its order within equal keys, and those items leave the round as
+// singletons (the L sort: successor an LMS suffix or the sentinel; the S
+// sort: successor an L suffix).  The labels and ties that come out equal
+// those of the full keys (rank(p), label(p+1)).  kpre (optional): the
+// round-1 keys of items[0..npre), run offset included at RUN_KEY_SHIFT
+// (the caller builds them where it has their symbols at hand).
 inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                        bool last_slot_labels, const u32 *rank = nullptr,
                        const u32 *tails = nullptr, u32 k = 0, bool giveup = false,
-                       u32 *SA = nullptr, const u32 *runbits = nullptr) {
+                       u32 *SA = nullptr, const u32 *runbits = nullptr, int final_mode = 0,
+                       const u64 *kpre = nullptr, u32 npre = 0) {
     Dev &d = rt.d;
     if (A == 0) return true;
     const u32 A0 = A;
@@ -136,6 +147,10 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         });
         Bh = bit_width(k - 1);
     }
+    const int fm = byrank && runbits ? final_mode : 0;
+    // round-1 key of the final sorts: rank(p) | rank(p+1) + 1 (0: the
+    // sentinel) | p+1 S-type | alone
+    const int B1 = fm ? bit_width(k) + 2 : B;
     u64 h = 1;
     int rounds = 0;
     u32 G = 0;             // tie groups entering the round (after round 1)
@@ -155,6 +170,8 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                         A < (1u << 28);
         const bool prevc = carried, sit = last_slot_labels;
         const u32 *rb = runbits;
+        const bool f1 = first && fm;
+        const int fmm = fm, BB1 = B1;
         for_each(d, A, [=] __device__(u64 i) {
             // round 1 reads the caller's items and makes the working copy
             u32 p;
@@ -164,13 +181,31 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             } else {
                 p = ac[i];
             }
-            u64 q = (u64)p + hh;
-            u32 lo = 0;
-            if (q <= mm) lo = RK[q];
-            else atomicOr(err, 16u);
+            u64 key;
             const u32 gv = first ? 0u : gi[i];
-            u32 hi = first ? (byrank ? rank[p] : RK[p]) : (prevc ? gv & RUN_GID_MASK : gv);
-            u64 key = ((u64)hi << B) | (u64)lo;
+            if (f1 && i < npre) {
+                key = kpre[i];
+                if (!cr) key &= (1ull << RUN_KEY_SHIFT) - 1;
+                keys[i] = key;
+                return;
+            }
+            if (f1) {
+                const u64 q = (u64)p + 1;
+                u32 c2 = 0, ob = 0;
+                if (q < mm) {
+                    c2 = rank[q] + 1;
+                    ob = getbit(rb, q);
+                }
+                const u32 so = fmm == 1 ? (q < mm ? ob : 1u) : (ob ^ 1u);
+                key = ((u64)rank[p] << BB1) | ((u64)c2 << 2) | ((u64)ob << 1) | so;
+            } else {
+                u64 q = (u64)p + hh;
+                u32 lo = 0;
+                if (q <= mm) lo = RK[q];
+                else atomicOr(err, 16u);
+                u32 hi = first ? (byrank ? rank[p] : RK[p]) : (prevc ? gv & RUN_GID_MASK : gv);
+                key = ((u64)hi << B) | (u64)lo;
+            }
             if (cr)
                 key |= (u64)(first ? run_offset(rb, p, sit) : gv >> RUN_GID_SHIFT)
                        << RUN_KEY_SHIFT;
@@ -178,7 +213,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         });
         // round 1 sorts everything; later rounds keep every tie group in its
         // place and sort it by the label bits alone
-        bool sw = first ? radix_sort_kv(d, A, keys, act, kt, vt, Bh + B)
+        bool sw = first ? radix_sort_kv(d, A, keys, act, kt, vt, Bh + (f1 ? B1 : B))
                         : group_sort_kv(d, A, G, gid, prevc ? RUN_GID_MASK : ~0u, keys, act, kt,
                                         vt, B);
         const u64 *K = sw ? kt : keys;
@@ -193,7 +228,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             u32 *tg = d.a->get<u32>(nt), *ts = d.a->get<u32>(nt);
             u32 *tk = d.a->get<u32>(nt), *te = d.a->get<u32>(nt);
             u32 *tq = d.a->get<u32>(nt);
-            RelIn in{K, AA, B, false, cr ? (1ull << RUN_KEY_SHIFT) - 1 : ~0ull};
+            RelIn in{K, AA, f1 ? B1 : B, false, cr ? (1ull << RUN_KEY_SHIFT) - 1 : ~0ull, f1};
             k_rel_reduce<<<nt, BLOCK, 0, d.s>>>(in, tg, ts, tk, te, tq);
             PSK_LAUNCH_CHECK();
             k_rel_tiles<<<lastconv ? 5 : 4, BLOCK, 0, d.s>>>(tg, ts, tk, tq, te, nt, cnt);
@@ -627,35 +662,83 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         d.phase(what);
     }
     labels_L(d, s, m, sbits, RK);
+    // The final L sort's round 1 takes short keys (chain_sort, final) when
+    // the ranks leave room for a run offset beside them: its items are the
+    // predecessors of the sorted LMS suffixes in that order, then the L items
+    // whose successor is L or the sentinel.  cnt is free once the LMS labels
+    // are placed and holds that item order.
+    const bool shortL = rank && m1 && (u64)k < (1ull << 27);
+    u32 *ordL = cnt;
+    u64 *kpre = shortL ? d.a->get<u64>(m1) : nullptr;
+    const int B1 = bit_width(k) + 2;
     // sorted LMS suffixes are grouped by first symbol already
     if (m1) {
-        // SA1 holds reduced indices: one 8-byte record (symbol, position) per
-        // LMS position in text order (sequential reads), then one record
-        // gather per sorted entry gives the list and its symbols
+        // SA1 holds reduced indices: one record per LMS position in text order
+        // (sequential reads), then one record gather per sorted entry gives
+        // the list and its symbols.  Records are (symbol, position), or with
+        // the short L keys (position, symbol, rank of the predecessor, rank + 1
+        // | the predecessor's L-run offset << 27)
         ArenaScope m2(*d.a);
-        // one block: R, vals, list
-        u64 *R = d.a->get<u64>(2ull * m1 + m1 / 2 + 1);
-        u64 *vals = R + m1;
-        u32 *list = (u32 *)(vals + m1);
-        for_each(d, m1, [=] __device__(u64 p) {
-            const u32 x = lms_text[p];
-            R[p] = ((u64)s[x] << 32) | x;
-        });
+        u64 *vals = d.a->get<u64>(m1);
+        u32 *list = d.a->get<u32>(m1 + 1);
         u32 *err = d.err;
         const u32 mm1 = m1;
-        for_each(d, m1, [=] __device__(u64 j) {
-            const u32 p = SA1[j];
-            u64 r = 0;
-            if (p < mm1) r = R[p];
-            else atomicOr(err, 512u);
-            list[j] = (u32)r;
-            vals[j] = r >> 32;
-        });
+        if (shortL) {
+            uint4 *R = d.a->get<uint4>(m1);
+            const u32 *sb = sbits;
+            for_each(d, m1, [=] __device__(u64 p) {
+                const u32 x = lms_text[p];  // an LMS position: x >= 1, x-1 L-type
+                const u32 off = run_offset(sb, x - 1, false);
+                R[p] = make_uint4(x, s[x], rank[x - 1], (rank[x] + 1) | (off << 27));
+            });
+            const int b1 = B1;
+            for_each(d, m1, [=] __device__(u64 j) {
+                const u32 p = SA1[j];
+                uint4 r = make_uint4(0, 0, 0, 0);
+                if (p < mm1) r = R[p];
+                else atomicOr(err, 512u);
+                list[j] = r.x;
+                vals[j] = r.y;
+                kpre[j] = ((u64)r.z << b1) | ((u64)(r.w & ((1u << 27) - 1)) << 2) | 3ull |
+                          ((u64)(r.w >> 27) << RUN_KEY_SHIFT);
+            });
+        } else {
+            u64 *R = d.a->get<u64>(m1);
+            for_each(d, m1, [=] __device__(u64 p) {
+                const u32 x = lms_text[p];
+                R[p] = ((u64)s[x] << 32) | x;
+            });
+            for_each(d, m1, [=] __device__(u64 j) {
+                const u32 p = SA1[j];
+                u64 r = 0;
+                if (p < mm1) r = R[p];
+                else atomicOr(err, 512u);
+                list[j] = (u32)r;
+                vals[j] = r >> 32;
+            });
+        }
         labels_lms_at_tails(d, s, list, vals, m1, RK, cnt);
+        if (shortL) for_each(d, m1, [=] __device__(u64 j) { ordL[j] = list[j] - 1; });
         m2.reset();
     }
     // every position is an L or an S item: the two sorts write the whole SA
-    chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
+    if (shortL) {
+        const u64 mm = m;
+        const u32 *Li = Litems;
+        scan_exclusive(
+            d, nL,
+            [=] __device__(u64 i) {
+                const u64 q = (u64)Li[i] + 1;
+                return (q >= mm || !getbit(sbits, q)) ? 1u : 0u;
+            },
+            [=] __device__(u64 i, u32 ex, u32 v) {
+                if (v) ordL[m1 + ex] = Li[i];
+            },
+            nullptr);
+        chain_sort(rt, m, RK, ordL, nL, false, rank, tails, k, false, SA, sbits, 1, kpre, m1);
+    } else {
+        chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
+    }
     labels_S(d, s, m, sbits, RK);
     chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, sbits);
     if (d.trace) {

@@ -24,15 +24,20 @@ struct RelIn {
     int B;
     bool tbit;  // bit 0 of a key set: the item is finished (never kept)
     u64 kmask = ~0ull;  // the sorted bits of a key (the rest rides along)
+    // bit 0 of a key set: the item is alone in its subgroup (finished) even
+    // where its neighbours carry the same key -- their order is already
+    // final (the sort was stable over input in final order)
+    bool solo = false;
     // flags of item i: group start, subgroup start, kept, group end
     __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
         u64 k = K[i] & kmask, hi = k >> B;
         u64 kp_ = i ? K[i - 1] & kmask : ~0ull;
         u64 kn_ = i + 1 < A ? K[i + 1] & kmask : ~0ull;
+        const bool so = solo && (k & 1u);
         gs = i == 0 || (kp_ >> B) != hi;
-        ss = i == 0 || kp_ != k;
+        ss = i == 0 || kp_ != k || so;
         ge = i + 1 == A || (kn_ >> B) != hi;
-        kp = (!ss || (i + 1 < A && kn_ == k)) && !(tbit && (k & 1u));
+        kp = (!ss || (i + 1 < A && kn_ == k)) && !(tbit && (k & 1u)) && !so;
     }
 };
 
