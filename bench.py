@@ -32,6 +32,10 @@ CONFIGS = {
     "iid64m": "plain psacak8, 64 MiB i.i.d. uniform ACGT",
     "zipf32": "psacak32 (translate + integer SACA), 256 Mi u32 Zipf(1.1) over 2^20 + 20% repeats",
     "batch10k": "batched psacak8, 10,000 i.i.d. DNA texts, lengths log-uniform 1 KiB-4 MiB",
+    # the reference's bench/generate.sh shapes: random bytes over k symbols
+    "k4": "plain GPU psacak8, random text over 4 symbols (bench/generate.sh random-k004)",
+    "k128": "plain GPU psacak8, random text over 128 symbols (bench/generate.sh random-k128)",
+    "k256": "plain GPU psacak8, random text over 256 symbols (bench/generate.sh random-k256)",
     "served": "served psacak8: Poisson 200 req/s, i.i.d. DNA texts log-uniform 10 KiB-64 MiB, "
               "1% 1-2 GiB, continuous batching under an HBM budget",
 }
@@ -50,7 +54,29 @@ def make_text(config: str, n: int | None):
     if config == "iid64m":
         n = n or (1 << 26)
         return gen.iid_dna(n, seed=20260102), n
+    if config in ("k4", "k128", "k256"):
+        # generate.sh maps random bytes i -> i % k (256 MiB and 1 GiB files)
+        n = n or (1 << 28)
+        k = int(config[1:])
+        return gen.uniform8(n, alphabet=k, seed=20260106 + k), n
     raise SystemExit(f"unknown config {config}")
+
+
+def host_cpu():
+    """Cores this process may use and the CPU model (for the CPU baseline)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except Exception:
+        usable = os.cpu_count() or 0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    return {"nproc": usable, "logical_cpus": os.cpu_count() or 0, "cpu_model": model}
 
 
 def level_bytes(rows):
@@ -128,6 +154,7 @@ def cpu_baseline(text, sample_bytes):
         "value": round(len(sample) / dt / 1e6, 3),
         "unit": "MB/s",
         "cores": threads,
+        "host": host_cpu(),
         "kind": "port",
         "sample": f"first {len(sample)} bytes of the same text; multi-threaded C restatement "
                   f"of psacak8 with the pipelined parallel induce (oracle/psk_par.c), "
@@ -191,7 +218,8 @@ def bench_zipf32(args):
         ref1 = oracle.sacak32_raw(text[:m])
         dt1 = time.perf_counter() - t2
         sub_sa = P.psacak32_raw(text[:m], ctx)
-        cpu = {"value": round(m * 4 / dt / 1e6, 3), "unit": "MB/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(m * 4 / dt / 1e6, 3), "unit": "MB/s", "cores": threads,
+               "host": host_cpu(), "kind": "port",
                "sample": f"first {m} symbols, multi-threaded C restatement (serial translate + "
                          f"psacak32 with the pipelined parallel induce), {threads} threads, "
                          f"{dt:.2f} s; GPU SA of the sample equal: "
@@ -508,8 +536,18 @@ def main():
         }
     alg = level_bytes(rows)
     dev_avg = sum(dev_ms) / len(dev_ms)
+    # (key, value) pairs moved by radix scatter passes per call, and the
+    # bytes the radix kernels account for (24 B per pair per scatter pass,
+    # 8 B per pair per counting pass)
+    rs = [k for k in kstats if k["name"] in ("k_rs_scatter", "k_gs_scatter", "k_rs_small")]
+    pair_passes = sum(k["bytes"] for k in rs) / 24.0 / args.steps
+    radix_bytes = sum(k["bytes"] for k in kstats
+                      if k["name"] in ("k_rs_scatter", "k_gs_scatter", "k_rs_small",
+                                       "k_rs_count", "k_gs_count")) / args.steps
     pipeline = {
         "algorithmic_bytes": int(alg), "device_ms": round(dev_avg, 3),
+        "radix_pair_passes_per_call": int(pair_passes),
+        "radix_bytes_per_call": int(radix_bytes),
         "achieved_GBps": round(alg / (dev_avg / 1000.0) / 1e9, 2),
         "frac": round(alg / (dev_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
         "levels": [[int(x) for x in r] for r in rows],
@@ -523,9 +561,11 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(text, min(args.cpu_sample, n))
 
+    metric = {"markov1g": "input MB/s at 1 GiB DNA (bit-exact SA)",
+              "iid64m": "input MB/s at 64 MiB DNA (bit-exact SA)"}.get(
+        args.config, f"input MB/s at {n >> 20} MiB random {args.config} text (bit-exact SA)")
     out = {
-        "metric": ("input MB/s at 1 GiB DNA (bit-exact SA)" if args.config == "markov1g"
-                   else "input MB/s at 64 MiB DNA (bit-exact SA)"),
+        "metric": metric,
         "value": round(mbps, 2),
         "unit": "MB/s",
         "n_gpus": 1,
@@ -537,7 +577,9 @@ def main():
         "vs_baseline": None,
         "dtype": "u8->u32",
         "data": "synthetic (seeded generator psacak_amd/csrc/gen.c)",
-        "config": {"workload": CONFIGS[args.config], "n": n, "seed": 20260101,
+        "config": {"workload": CONFIGS[args.config], "n": n,
+                   "seed": {"markov1g": 20260101, "iid64m": 20260102}.get(
+                       args.config, 20260106 + int(args.config[1:]) if args.config[0] == "k" else 0),
                    "sa": "u32", "path": "psk_psacak8_dev, text resident in HBM"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -223,6 +223,12 @@ int psk_server_wait(psk_server *s, uint64_t id, uint32_t *sa, double *latency_ms
 uint64_t psk_server_peak_bytes(psk_server *s);
 /* Number of requests submitted but not yet finished. */
 uint64_t psk_server_pending(psk_server *s);
+/* Counters since creation, out[0..min(n,7)): jobs admitted, requests in
+ * them, admission checks that found the head request blocked by the budget,
+ * workspace trims, the most estimated footprint in flight at once (bytes),
+ * requests admitted out of arrival order (0 by design), and the peak device
+ * bytes reserved.  Returns the number written. */
+int psk_server_stats(psk_server *s, uint64_t *out, int n);
 void psk_server_destroy(psk_server *s);
 
 #ifdef __cplusplus

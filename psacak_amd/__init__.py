@@ -96,6 +96,7 @@ def _load():
         "psk_server_wait": ([vp, u64, vp, ctypes.POINTER(ctypes.c_double), ctypes.c_double], i32),
         "psk_server_peak_bytes": ([vp], u64),
         "psk_server_pending": ([vp], u64),
+        "psk_server_stats": ([vp, vp, i32], i32),
         "psk_server_destroy": ([vp], None),
         "psk_profile_stats": ([vp, ctypes.c_char_p, u64, vp, vp, vp, i32], i32),
         "psk_psacak8_dev_noalloc": ([vp, u64, vp, vp], i32),
@@ -500,6 +501,15 @@ class Server:
 
     def pending(self) -> int:
         return int(_lib.psk_server_pending(self._h))
+
+    def stats(self) -> dict:
+        """Admission counters (psk_server_stats)."""
+        out = np.zeros(7, dtype=np.uint64)
+        n = _lib.psk_server_stats(self._h, out.ctypes.data, 7)
+        _raise(n if n < 0 else PSK_OK)
+        keys = ["jobs", "admitted", "blocked", "trims", "max_inflight", "order_violations",
+                "peak_bytes"]
+        return {k: int(v) for k, v in zip(keys, out[:n])}
 
     def close(self):
         if getattr(self, "_h", None):

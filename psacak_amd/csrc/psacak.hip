@@ -713,12 +713,20 @@ static void server_worker(psk_server *s, int w) {
                     }
                     if (admit) {
                         for (size_t i = 0; i < cand.size(); i++) s->pending.pop_front();
+                        for (SrvReq *r : cand) {
+                            if (r->id <= s->last_admitted) s->order_violations++;
+                            s->last_admitted = r->id;
+                        }
+                        s->n_jobs++;
+                        s->n_admitted += cand.size();
                         s->inflight += f;
+                        s->max_inflight = std::max(s->max_inflight, s->inflight);
                         s->wfoot[w] = f;
                         foot = f;
                         job.swap(cand);
                         break;
                     }
+                    s->n_blocked++;
                     if (mine) {
                         // idle and blocked: give back the cache (below, with
                         // the lock released), then look again
@@ -732,6 +740,10 @@ static void server_worker(psk_server *s, int w) {
             }
         }
         if (trim) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->n_trims++;
+            }
             (void)psk_ctx_trim(c);
             if (job.empty()) {
                 s->cv_work.notify_all();
@@ -896,6 +908,16 @@ uint64_t psk_server_peak_bytes(psk_server *s) {
     if (!s) return 0;
     std::lock_guard<std::mutex> lk(s->mu);
     return s->peak;
+}
+
+int psk_server_stats(psk_server *s, uint64_t *out, int n) {
+    if (!s || !out || n < 0) return PSK_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const u64 v[7] = {s->n_jobs, s->n_admitted, s->n_blocked, s->n_trims,
+                      s->max_inflight, s->order_violations, s->peak};
+    const int m = std::min(n, 7);
+    for (int i = 0; i < m; i++) out[i] = v[i];
+    return m;
 }
 
 uint64_t psk_server_pending(psk_server *s) {

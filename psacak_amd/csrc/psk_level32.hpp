@@ -300,8 +300,12 @@ inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
 // gets slot s[x] = tail-1, the one before it tail-2, ...  vals: the symbol of
 // each list item when the caller has it (sorted keys), else nullptr (one
 // gather per item).  grp: scratch of m+1 u32.
+// need (optional, one key per list item): only items whose key carries a
+// run offset >= 1 at RUN_KEY_SHIFT get their label written (the final L sort
+// with short round-1 keys reads an LMS label only through a chain of at
+// least two L positions before it).
 inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64 *vals, u64 cnt,
-                                u32 *RK, u32 *grp) {
+                                u32 *RK, u32 *grp, const u64 *need = nullptr) {
     if (cnt == 0) return;
     u32 *err = d.err;
     ArenaScope mk(*d.a);
@@ -317,6 +321,7 @@ inline void labels_lms_at_tails(Dev &d, const u32 *s, const u32 *list, const u64
         }
     });
     for_each(d, cnt, [=] __device__(u64 j) {
+        if (need && (need[j] >> RUN_KEY_SHIFT) == 0) return;
         u32 v = vals ? (u32)vals[j] : vb[j];
         u32 back = grp[v] - (u32)j;
         if (back <= v) RK[list[j]] = v - back + 1;
@@ -717,7 +722,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
                 vals[j] = r >> 32;
             });
         }
-        labels_lms_at_tails(d, s, list, vals, m1, RK, cnt);
+        labels_lms_at_tails(d, s, list, vals, m1, RK, cnt, kpre);
         if (shortL) for_each(d, m1, [=] __device__(u64 j) { ordL[j] = list[j] - 1; });
         m2.reset();
     }

@@ -55,6 +55,11 @@ struct psk_server {
     std::vector<std::thread> workers;
     std::vector<psk_ctx *> ctxs;
     std::vector<psk::u64> wfoot;  // footprint of each worker's running job (0: idle)
+    // counters (psk_server_stats): jobs admitted, requests in them, times a
+    // worker found the head request blocked by the budget, trims, the most
+    // footprint in flight at once, and admissions out of arrival order
+    psk::u64 n_jobs = 0, n_admitted = 0, n_blocked = 0, n_trims = 0, max_inflight = 0,
+             order_violations = 0, last_admitted = 0;
     // requests up to this size are coalesced; a coalesced batch holds at
     // most kBatchBytes of text
     static constexpr psk::u64 kSmall = 16ull << 20;

@@ -187,6 +187,8 @@ def test_server_concurrent_waits_on_one_id():
         done = ctypes.c_int(0)
         assert P._lib.psk_server_poll(s._h, rid, ctypes.byref(done), None, None) == P.PSK_ENOTFOUND
         assert s.pending() == 0
+        st = s.stats()
+        assert st["admitted"] == 1 and st["order_violations"] == 0
 
 
 def test_default_context_trim_and_workspace():

@@ -654,3 +654,68 @@ def test_level_intermediates_psacak32_zipf():
     assert rows[0, 3] == k0
     assert cap is not None and np.array_equal(cap, red0)
     assert np.array_equal(sa, oracle.sacak32(tr))
+
+
+def test_server_under_a_binding_budget(ctx):
+    # a budget that holds the large request's footprint but not much else:
+    # admission must wait for memory (blocked checks), stay in arrival order,
+    # keep the reserved device memory inside the budget, and every result
+    # must equal the same text's SA computed alone
+    note("server binding budget")
+    rng = np.random.default_rng(23)
+    small = [gen.markov_dna(int(n), seed=400 + i)
+             for i, n in enumerate(gen.lengths(60, 10 << 10, 2 << 20, seed=44))]
+    big = gen.iid_dna(64 << 20, seed=45)
+    texts = small[:30] + [big] + small[30:]
+    alone = [P.psacak(t, ctx) for t in texts]
+    budget = int(P.Server.footprint(len(big)) * 1.15)
+    with P.Server(hbm_budget=budget, workers=3) as srv:
+        ids = []
+        for i, t in enumerate(texts):
+            ids.append(srv.submit(t) if i % 2 else srv.submit_into(t))
+            if rng.random() < 0.3:
+                import time
+                time.sleep(0.001)
+        for i, rid in enumerate(ids):
+            sa, _ = srv.wait(rid, timeout_ms=120000)
+            assert np.array_equal(sa, alone[i]), i
+        st = srv.stats()
+        assert srv.pending() == 0
+    assert st["admitted"] == len(texts)
+    assert st["order_violations"] == 0
+    assert st["blocked"] > 0, st        # the budget did bind
+    assert st["max_inflight"] <= budget, st
+    assert st["peak_bytes"] <= budget, st
+    for i in (0, 30, 45):
+        assert np.array_equal(alone[i], oracle.sacak8(texts[i]))
+
+
+def test_text_above_2_31_bytes(ctx):
+    # the crate's domain runs to n < 2^32 - 1 (lib.rs:65,79): a text of
+    # 2^31 + 2^20 bytes (level 1 then has about 6.4*10^8 symbols), verified on
+    # the device (permutation + adjacent-pair rank condition) and by direct
+    # comparison of sampled adjacent suffixes
+    import torch
+    n = (1 << 31) + (1 << 20)
+    note(f"text of {n} bytes")
+    t = gen.iid_dna(n, seed=31)
+    dt = torch.from_numpy(t).to("cuda:0")
+    ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    P.psacak8_dev(dt, ds, ctx=ctx)
+    torch.cuda.synchronize()
+    assert P.check8_dev(dt, ds, ctx=ctx)
+    lv = ctx.last_levels()
+    assert lv[0, 0] == n and lv.shape[0] >= 2
+    rng = np.random.default_rng(5)
+    idx = rng.integers(1, n, 256)
+    sa_lo = ds[torch.from_numpy(idx - 1).to("cuda:0")].cpu().numpy().view(np.uint32)
+    sa_hi = ds[torch.from_numpy(idx).to("cuda:0")].cpu().numpy().view(np.uint32)
+    tb = memoryview(t)
+    for a, b in zip(sa_lo, sa_hi):
+        a, b = int(a), int(b)
+        x, y = bytes(tb[a:a + 4096]), bytes(tb[b:b + 4096])
+        assert x != y, (a, b)
+        k = next(j for j in range(min(len(x), len(y))) if x[j] != y[j])
+        assert x[k] < y[k], (a, b)
+    del dt, ds
+    torch.cuda.empty_cache()
