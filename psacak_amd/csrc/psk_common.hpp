@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -113,6 +114,24 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T *lds, T *total) {
     return r;
 }
 
+// ---- fault injection ------------------------------------------------------------
+// PSACAK_AMD_FAIL_ALLOC=k (k >= 1) in the environment makes every k-th
+// workspace chunk allocation of the process fail as if hipMalloc had (the
+// count restarts whenever the value changes): for tests that every failure
+// surfaces as a status code and leaves the context usable.  Unset or 0: off.
+inline bool inject_alloc_failure() {
+    static std::atomic<long> count{0};
+    static std::atomic<long> period{0};
+    const char *e = getenv("PSACAK_AMD_FAIL_ALLOC");
+    const long k = e ? atol(e) : 0;
+    if (k != period.load()) {
+        period.store(k);
+        count.store(0);
+    }
+    if (k <= 0) return false;
+    return (count.fetch_add(1) + 1) % k == 0;
+}
+
 // ---- device arena -------------------------------------------------------------
 // Stack allocator over large hipMalloc'd chunks that are kept for reuse.
 // mark()/reset() pop everything allocated after the mark; ArenaScope does it
@@ -175,6 +194,8 @@ class Arena {
                 fail(PSK_EBUDGET, "device workspace exceeds the HBM budget");
         }
         char *p = nullptr;
+        if (inject_alloc_failure())
+            fail(PSK_ENOMEM, "hipMalloc of " + std::to_string(cap) + " bytes failed (injected)");
         if (hipMalloc((void **)&p, cap) != hipSuccess) {
             (void)hipGetLastError();
             fail(PSK_ENOMEM, "hipMalloc of " + std::to_string(cap) + " bytes failed");

@@ -36,7 +36,10 @@ struct Xfer {
     }
     void ready() {
         for (int i = 0; i < 2; i++) {
-            if (!stage[i]) PSK_HIP(hipHostMalloc((void **)&stage[i], XFER_CHUNK, hipHostMallocDefault));
+            if (!stage[i]) {
+                if (inject_alloc_failure()) fail(PSK_ENOMEM, "pinned staging allocation failed (injected)");
+                PSK_HIP(hipHostMalloc((void **)&stage[i], XFER_CHUNK, hipHostMallocDefault));
+            }
             if (!ev[i]) PSK_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
         }
     }

@@ -719,3 +719,46 @@ def test_text_above_2_31_bytes(ctx):
         assert x[k] < y[k], (a, b)
     del dt, ds
     torch.cuda.empty_cache()
+
+
+def test_injected_allocation_failures_leave_contexts_usable():
+    # every k-th workspace allocation fails (PSACAK_AMD_FAIL_ALLOC=k): each
+    # entry point returns a status code (no abort, no wrong SA), and once the
+    # injection stops the same context computes correct SAs again
+    import torch
+    note("fault injection")
+    t = gen.markov_dna(6 << 20, seed=51)
+    exp = oracle.sacak8(t)
+    z = gen.zipf32(1 << 18, sigma=1 << 14, seed=52)
+    expz = oracle.sacak32_raw(z)
+    dt = torch.from_numpy(t).to("cuda:0")
+    ds = torch.empty(len(t), dtype=torch.int32, device="cuda:0")
+    failures = 0
+    try:
+        for k in (1, 2, 3, 5, 8):
+            with P.Context() as c:
+                os.environ["PSACAK_AMD_FAIL_ALLOC"] = str(k)
+                for call in ("dev", "host", "raw32", "batch", "check"):
+                    try:
+                        if call == "dev":
+                            P.psacak8_dev(dt, ds, ctx=c)
+                        elif call == "host":
+                            P.psacak(t, c)
+                        elif call == "raw32":
+                            P.psacak32_raw(z, c)
+                        elif call == "batch":
+                            P.batch8([t[:100000], t[100000:300000]], c)
+                        else:
+                            P.check8_dev(dt, ds, ctx=c)
+                    except P.PskError as e:
+                        assert e.code in (P.PSK_ENOMEM, P.PSK_EBUDGET), (k, call, e)
+                        failures += 1
+                os.environ["PSACAK_AMD_FAIL_ALLOC"] = "0"
+                P.psacak8_dev(dt, ds, ctx=c)
+                torch.cuda.synchronize()
+                assert np.array_equal(ds.cpu().numpy().view(np.uint32), exp), k
+                assert np.array_equal(P.psacak(t, c), exp), k
+                assert np.array_equal(P.psacak32_raw(z, c), expz), k
+    finally:
+        os.environ.pop("PSACAK_AMD_FAIL_ALLOC", None)
+    assert failures > 0
