@@ -473,9 +473,12 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dev_ms = []
+    syncs = []
     for _ in range(args.steps):
         P.psacak8_dev(d_text, d_sa, n, ctx=ctx, stream=stream)
-        dev_ms.append(float(ctx.last_timing_ms()[0]))
+        tm = ctx.last_timing_ms()
+        dev_ms.append(float(tm[0]))
+        syncs.append(int(tm[1]))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kstats = P.kernel_stats(ctx)
@@ -547,6 +550,7 @@ def main():
     pipeline = {
         "algorithmic_bytes": int(alg), "device_ms": round(dev_avg, 3),
         "radix_pair_passes_per_call": int(pair_passes),
+        "host_syncs_per_call": max(syncs) if syncs else None,
         "radix_bytes_per_call": int(radix_bytes),
         "achieved_GBps": round(alg / (dev_avg / 1000.0) / 1e9, 2),
         "frac": round(alg / (dev_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),

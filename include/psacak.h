@@ -156,8 +156,10 @@ int psk_ctx_set_capture(psk_ctx *ctx, int level);
  * recursion ended above it). */
 int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *count);
 
-/* Time (ms, HIP events) of the last call's device work, by phase:
- * [0] total, [1] level-0 induce, [2] deeper levels, [3] naming+typing. */
+/* The last call on ctx: [0] time of its device work (ms, HIP events around
+ * the call on its stream), [1] host synchronisations with the stream inside
+ * the call (readbacks of small counters that size later launches), [2], [3]
+ * reserved (0). */
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
 
 /* ---- kernel profiling ---------------------------------------------------- */

@@ -26,6 +26,7 @@ struct psk_ctx {
     Prof prof;
     Xfer xfer;
     double last_ms[4] = {0, 0, 0, 0};
+    u64 last_syncs = 0;  // host synchronisations inside the last timed call
 };
 
 namespace {
@@ -98,7 +99,8 @@ int check32(const void *text, u64 n, const void *sa, u64 sa_len) {
 
 struct Timer {
     psk_ctx *c;
-    explicit Timer(psk_ctx *cc) : c(cc) {
+    u64 syncs0;
+    explicit Timer(psk_ctx *cc) : c(cc), syncs0(cc->rt.d.syncs) {
         c->rt.d.zero(c->rt.d.err, sizeof(u32));
         PSK_HIP(hipEventRecord(c->ev0, c->stream));
     }
@@ -108,8 +110,18 @@ struct Timer {
         float ms = 0;
         PSK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->last_ms[0] = ms;
-        u32 e = c->rt.d.read_u32(c->rt.d.err);
-        if (e) fail(PSK_EDEVICE, "internal index check failed (flags " + std::to_string(e) + ")");
+        c->last_syncs = c->rt.d.syncs - syncs0;
+        if (c->rt.d.trace)
+            fprintf(stderr, "[psk] call: %.3f ms, %llu host synchronisations\n", ms,
+                    (unsigned long long)c->last_syncs);
+        // the index-check flag and each level's first LMS position, in one read
+        u32 w[64];
+        c->rt.d.read(w, c->rt.d.err, sizeof(w));
+        if (w[0]) fail(PSK_EDEVICE, "internal index check failed (flags " + std::to_string(w[0]) + ")");
+        for (size_t i = 0; i < c->rt.levels.size() && i < 62; i++) {
+            LevelStat &s = c->rt.levels[i];
+            if (s.m1 && !s.sumlen) s.sumlen = s.m - w[1 + i] + s.m1;
+        }
     }
 };
 
@@ -471,7 +483,8 @@ int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries) {
     if (!ctx || !ms) return PSK_EINVAL;
     int n = std::min(max_entries, 4);
-    for (int i = 0; i < n; i++) ms[i] = ctx->last_ms[i];
+    const double v[4] = {ctx->last_ms[0], (double)ctx->last_syncs, 0.0, 0.0};
+    for (int i = 0; i < n; i++) ms[i] = v[i];
     return n;
 }
 

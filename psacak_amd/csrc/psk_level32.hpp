@@ -33,6 +33,15 @@ struct LevelStat {
     u64 m, m1, mL, k, sumlen, csize;
 };
 
+// The first LMS position of a level goes to the call's deferred statistics
+// (err[1 + depth]); the level's sum of LMS-substring lengths follows from it
+// when the call ends, without a readback of its own.
+inline void record_first_lms(Dev &d, const u32 *lms_text, u32 m1, int depth) {
+    if (!m1 || depth >= 62) return;
+    u32 *slot = d.err + 1 + depth;
+    for_each(d, 1, [=] __device__(u64) { *slot = lms_text[0]; });
+}
+
 struct Runtime {
     Dev d;
     std::vector<LevelStat> levels;
@@ -589,11 +598,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     d.zero(ctr, 2 * sizeof(u32));
     k_lms32<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(sbits, lmsbits, nwords, ctr);
     PSK_LAUNCH_CHECK();
-    const u32 m1 = d.read_u32(ctr);
-    u32 *lms_text = d.a->get<u32>(m1 + 1);
-    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
-
-    // L and S item lists (ascending positions)
+    // L item list (ascending positions); then the LMS and L counts in one read
     u32 *Litems = d.a->get<u32>(m);
     u32 *dcnt = ctr + 1;
     scan_exclusive(
@@ -602,8 +607,12 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
             if (v) Litems[off] = (u32)x;
         },
         dcnt);
-    const u32 nL = d.read_u32(dcnt);
+    u32 cts[2];
+    d.read(cts, ctr, sizeof(cts));
+    const u32 m1 = cts[0], nL = cts[1];
     const u32 nS = (u32)m - nL;
+    u32 *lms_text = d.a->get<u32>(m1 + 1);
+    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
     u32 *Sitems = Litems + nL;
     positions_of(d, m, [=] __device__(u64 w) { return sbits[w]; }, Sitems);
 
@@ -615,11 +624,7 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     u32 *cnt = d.a->get<u32>(m + 1);
 
     // ---- LMS-substring names and the reduced problem ------------------------
-    if (m1) {
-        u32 first = 0;
-        d.read(&first, lms_text, sizeof(u32));
-        st.sumlen = m - first + m1;
-    }
+    record_first_lms(d, lms_text, m1, depth);
     rt.levels[depth] = st;
     u32 *SA1 = d.a->get<u32>(m1 + 1);
     // by sorting the substrings: symbols ranked by the names of the level

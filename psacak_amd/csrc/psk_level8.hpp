@@ -91,6 +91,32 @@ __global__ void __launch_bounds__(BLOCK) k_place_lms(const u32 *SA1, u32 m1, con
     }
 }
 
+// Level-0 tables from the bucket statistics (count, L count, LMS count per
+// byte), one workgroup: bucket heads and tails, zeroed induce counters, the
+// LMS placement bases and offsets, the start of each byte's LMS range in the
+// sorted LMS list (lstart, 257 entries) and the rank of each byte among those
+// present (crank).  Keeps the tables off the host.
+__global__ void __launch_bounds__(BLOCK) k_level0_tables(const u32 *stat, u32 *tables, u8 *crank,
+                                                         u32 *lstart) {
+    __shared__ u32 red[NWAVES + 1];
+    const u32 c = threadIdx.x;
+    const u32 cnt = stat[c], nlms = stat[512 + c];
+    u32 tot;
+    const u32 head = block_exclusive_sum(cnt, red, &tot);
+    const u32 ls = block_exclusive_sum(nlms, red, &tot);
+    const u32 rk = block_exclusive_sum(cnt ? 1u : 0u, red, &tot);
+    tables[c] = head;
+    tables[256 + c] = head + cnt;
+    tables[512 + c] = 0;   // fill
+    tables[768 + c] = 0;   // counts
+    tables[1024 + c] = 0;  // tmp
+    tables[1280 + c] = head + cnt - nlms;       // LMS placement base
+    tables[1536 + c] = head + cnt - nlms - ls;  // sorted LMS j -> slot j + lmsoff
+    lstart[c] = ls;
+    if (c == 255) lstart[256] = ls + nlms;
+    crank[c] = (u8)rk;
+}
+
 // SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
 // batch of texts, each with its own sentinel, and SA receives the SA of the
 // concatenation in which every text's suffixes are in their own order.
@@ -139,23 +165,15 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         nL_total += B.nL[c];
     }
 
-    // device tables: head, tail, fill, counts, tmp, lmsbase, lmsoff
+    // device tables: head, tail, fill, counts, tmp, lmsbase, lmsoff; the LMS
+    // ranges and the byte ranks
     u32 *tables = d.a->get<u32>(7 * 256 + 8);
     Induce8Tables tb{tables, tables + 256, tables + 512, tables + 768, tables + 1024};
     u32 *lmsbase = tables + 1280, *lmsoff = tables + 1536;
-    {
-        u32 h[7 * 256 + 8];
-        memset(h, 0, sizeof(h));
-        u64 start = 0;
-        for (int c = 0; c < 256; c++) {
-            h[c] = (u32)B.head[c];
-            h[256 + c] = (u32)B.head[c + 1];
-            h[1280 + c] = (u32)(B.head[c + 1] - B.nLMS[c]);
-            h[1536 + c] = (u32)(B.head[c + 1] - B.nLMS[c] - start);
-            start += B.nLMS[c];
-        }
-        d.write(tables, h, sizeof(h));
-    }
+    u32 *lstart = d.a->get<u32>(257);
+    u8 *crank = d.a->get<u8>(256);
+    k_level0_tables<<<1, BLOCK, 0, d.s>>>(stat, tables, crank, lstart);
+    PSK_LAUNCH_CHECK();
 
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
@@ -166,11 +184,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     d.phase("level 0 types, buckets, LMS list");
     LevelStat st{n, m1, nL_total, 0, 0, 1};
     rt.levels.resize(1);
-    if (m1) {
-        u32 first = 0;
-        d.read(&first, lms_text, sizeof(u32));
-        st.sumlen = n - first + m1;
-    }
+    record_first_lms(d, lms_text, m1, 0);
     rt.levels[0] = st;
 
     // ---- LMS-substring names and the reduced problem (psacak8.rs:33-48) ------
@@ -192,14 +206,8 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             ends.EB = EB;
             ends.EBR = EBR;
         }
-        u8 hcr[256];
         u32 sigma = 0;
-        for (int c = 0; c < 256; c++) {
-            hcr[c] = (u8)sigma;
-            if (B.cnt[c]) sigma++;
-        }
-        u8 *crank = d.a->get<u8>(256);
-        d.write(crank, hcr, sizeof(hcr));
+        for (int c = 0; c < 256; c++) sigma += B.cnt[c] != 0;
         named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st,
                                      ends);
     }
@@ -257,15 +265,6 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // SA1 is sorted, so its entries come grouped by first byte: the bucket of
     // entry j follows from the LMS counts, without reading T[p]
     if (m1) {
-        u32 *lstart = d.a->get<u32>(257);
-        u32 h[257];
-        u64 acc = 0;
-        for (int c = 0; c < 256; c++) {
-            h[c] = (u32)acc;
-            acc += B.nLMS[c];
-        }
-        h[256] = (u32)acc;
-        d.write(lstart, h, sizeof(h));
         // SA1 holds reduced indices (the recursion leaves the mapping to
         // this level)
         u64 blocks = std::min<u64>(ceil_div(m1, BLOCK), 65536);

@@ -29,14 +29,17 @@ struct Dev {
     u64 *pslot(int r) { return prof ? prof->dyn_slot(r) : nullptr; }
     void padd(int r, double bytes) { if (prof) prof->add(r, bytes); }
 
+    u64 syncs = 0;  // host synchronisations with the stream (readbacks, staged writes)
     u32 read_u32(const u32 *d) {
         PSK_HIP(hipMemcpyAsync(pinned, d, sizeof(u32), hipMemcpyDeviceToHost, s));
         PSK_HIP(hipStreamSynchronize(s));
+        syncs++;
         return pinned[0];
     }
     void read(void *host, const void *d, u64 bytes) {
         PSK_HIP(hipMemcpyAsync(pinned, d, bytes, hipMemcpyDeviceToHost, s));
         PSK_HIP(hipStreamSynchronize(s));
+        syncs++;
         memcpy(host, pinned, bytes);
     }
     void write(void *d, const void *host, u64 bytes) {
@@ -44,11 +47,13 @@ struct Dev {
         memcpy(pinned + 2048, host, bytes);
         PSK_HIP(hipMemcpyAsync(d, pinned + 2048, bytes, hipMemcpyHostToDevice, s));
         PSK_HIP(hipStreamSynchronize(s));
+        syncs++;
     }
     // arbitrary-size host -> device copy (pageable source), synchronous
     void upload(void *d, const void *host, u64 bytes) {
         if (bytes) PSK_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, s));
         PSK_HIP(hipStreamSynchronize(s));
+        syncs++;
     }
     void zero(void *d, u64 bytes) {
         if (bytes) PSK_HIP(hipMemsetAsync(d, 0, bytes, s));
