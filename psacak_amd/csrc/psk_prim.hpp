@@ -501,11 +501,17 @@ constexpr u64 RS_TILE = (u64)BLOCK * RS_ROUNDS;  // 2048 pairs per tile
 // Counting: one block per RS_CT consecutive tiles (fewer, longer-lived
 // workgroups; each digit's RS_CT counts are written as one contiguous run).
 constexpr u32 RS_CT = PSK_RS_CT;
+constexpr int RS_COPIES = 8;  // LDS copies of a tile's digit counts (power of two)
 __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, u64 n, int shift,
                                                     u32 *hist, u32 nt) {
-    __shared__ u32 h[RS_CT][256];
+    // eight copies of each tile's counts (lane & 7), padded onto different
+    // banks: lanes of a wave that share a digit (grouped keys) spread their
+    // atomics over eight counters instead of serialising on one
+    __shared__ u32 h[RS_CT][RS_COPIES][257];
     #pragma unroll
-    for (u32 c = 0; c < RS_CT; c++) h[c][threadIdx.x] = 0;
+    for (u32 c = 0; c < RS_CT; c++)
+        #pragma unroll
+        for (int q = 0; q < RS_COPIES; q++) h[c][q][threadIdx.x] = 0;
     const u32 ng = (nt + RS_CT - 1) / RS_CT;
     const u32 t0 = xcd_tile(blockIdx.x, ng) * RS_CT;
     __syncthreads();
@@ -528,14 +534,18 @@ __global__ void __launch_bounds__(BLOCK) k_rs_count(const u64 *__restrict__ ki, 
             const u32 x = dk[c][r];
             const u32 x0 = __shfl(x, 0, WAVE);
             if (ballot(x == x0) == ~0ull) {
-                if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[c][x0], (u32)WAVE);
+                if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[c][0][x0], (u32)WAVE);
             } else if (x < 256u) {
-                atomicAdd(&h[c][x], 1u);
+                atomicAdd(&h[c][lane_id() & (RS_COPIES - 1)][x], 1u);
             }
         }
     __syncthreads();
-    for (u32 c = 0; c < RS_CT && t0 + c < nt; c++)
-        hist[(u64)threadIdx.x * nt + t0 + c] = h[c][threadIdx.x];
+    for (u32 c = 0; c < RS_CT && t0 + c < nt; c++) {
+        u32 sum = 0;
+        #pragma unroll
+        for (int q = 0; q < RS_COPIES; q++) sum += h[c][q][threadIdx.x];
+        hist[(u64)threadIdx.x * nt + t0 + c] = sum;
+    }
 }
 
 // off: exclusive scan of the digit-major count matrix (256*nt + 1 entries,
@@ -883,8 +893,9 @@ __global__ void __launch_bounds__(BLOCK) k_gs_local(const u64 *K, const u32 *V, 
 template <class Tiles>
 __global__ void __launch_bounds__(BLOCK) k_gs_count(const u64 *__restrict__ ki, int shift,
                                                     u32 *hist, u32 nt, Tiles tiles) {
-    __shared__ u32 h[256];
-    h[threadIdx.x] = 0;
+    __shared__ u32 h[RS_COPIES][257];  // copies as in k_rs_count
+    #pragma unroll
+    for (int q = 0; q < RS_COPIES; q++) h[q][threadIdx.x] = 0;
     const u32 t = xcd_tile(blockIdx.x, nt);
     u64 base, mbase, ob;
     u32 cnt, ms;
@@ -895,13 +906,16 @@ __global__ void __launch_bounds__(BLOCK) k_gs_count(const u64 *__restrict__ ki, 
         const u32 x = j < cnt ? (u32)(ki[base + j] >> shift) & 255u : 256u;
         const u32 x0 = __shfl(x, 0, WAVE);
         if (ballot(x == x0) == ~0ull) {
-            if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[x0], (u32)WAVE);
+            if (lane_id() == 0 && x0 < 256u) atomicAdd(&h[0][x0], (u32)WAVE);
         } else if (x < 256u) {
-            atomicAdd(&h[x], 1u);
+            atomicAdd(&h[lane_id() & (RS_COPIES - 1)][x], 1u);
         }
     }
     __syncthreads();
-    hist[mbase + (u64)threadIdx.x * ms] = h[threadIdx.x];
+    u32 sum = 0;
+    #pragma unroll
+    for (int q = 0; q < RS_COPIES; q++) sum += h[q][threadIdx.x];
+    hist[mbase + (u64)threadIdx.x * ms] = sum;
 }
 
 // Stable scatter of one group tile (the ranking of k_rs_scatter): off holds

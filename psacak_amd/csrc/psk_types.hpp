@@ -333,9 +333,13 @@ constexpr u64 H_F = 21;  // bits per packed LDS counter field
 // text, so it is never LMS.
 __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, u64 nwords,
                             u32 *cnt, u32 *nL, u32 *nLMS, u32 *m1, const u32 *startbits) {
-    __shared__ unsigned long long h[256];
+    // four copies (lane & 3), padded onto different banks: a DNA text's
+    // wave holds four byte values, whose lanes would otherwise serialise
+    // on four counters
+    __shared__ unsigned long long h[4][257];
     __shared__ u32 red[NWAVES + 1];
-    h[threadIdx.x] = 0;
+    #pragma unroll
+    for (int q = 0; q < 4; q++) h[q][threadIdx.x] = 0;
     __syncthreads();
     u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x;
     u32 lcount = 0;
@@ -351,14 +355,14 @@ __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, 
             u32 c = t[p0 + b];
             unsigned long long v = 1ull | ((unsigned long long)(~(S >> b) & 1u) << H_F) |
                                    ((unsigned long long)((lms >> b) & 1u) << (2 * H_F));
-            atomicAdd(&h[c], v);
+            atomicAdd(&h[lane_id() & 3][c], v);
         }
     }
     u32 tot;
     block_exclusive_sum(lcount, red, &tot);
     if (threadIdx.x == 0 && tot) atomicAdd(m1, tot);
     __syncthreads();
-    unsigned long long v = h[threadIdx.x];
+    unsigned long long v = h[0][threadIdx.x] + h[1][threadIdx.x] + h[2][threadIdx.x] + h[3][threadIdx.x];
     if (v) {
         const unsigned long long M = (1ull << H_F) - 1;
         atomicAdd(&cnt[threadIdx.x], (u32)(v & M));
