@@ -184,71 +184,150 @@ __device__ __forceinline__ u64 chain_window(u64 w, u32 r, u32 c, bool has_sb, u3
 // and its predecessor byte in dig[0..limit), or `none` if it has no
 // predecessor: 0 for the L-scan, 255 for the S-scan, a byte that the scan
 // never induces from (0 is never > c, 255 never < c).  d_tmp: 2 u32.
+// A seed with a chain (r > 0): its position, run and window.
+struct ChainSeed {
+    u32 x, r;
+    u64 w;
+};
+
+// The run of c's before seed x (window w, refilled if it had run out) and
+// the byte before the run: sb | 256 if x - r has a predecessor, else 0.
+__device__ __forceinline__ u32 seed_run(const Pred &pr, u32 x, u64 w, u32 c, u32 *sb) {
+    const u32 v = Win::v(w);
+    *sb = 0;
+    if (!v || Win::b(w, 1) != c) return 0;
+    // inside the window, or beyond it from the decided bits
+    u32 r = 1;
+    while (r < v && Win::b(w, r + 1) == c) r++;
+    if (r < v) {
+        *sb = Win::b(w, r + 1) | 256u;
+    } else if (!Win::e(w)) {
+        r = pr.run_before(x);
+        if (pr.has(x - r)) *sb = pr.byte(x - r) | 256u;
+    }
+    return r;
+}
+
+constexpr int CQ_ROUNDS = 16;
+constexpr u64 CQ_TILE = (u64)BLOCK * CQ_ROUNDS;  // queue seeds per tile
+
+// Runs of one tile of seeds [t CQ_TILE, ...) of a queue with a seeds: every
+// seed's predecessor byte goes to dig (`none` without one), and the seeds
+// with a chain are compacted, in seed order, to the start of the tile's own
+// range of cs; tact[t] counts them and tent[t] their chain entries.
+// d_a (optional): the seed count on the device, taken up to a (the queue's
+// size) in place of a.
+__global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, const u32 *SA, u64 *PW,
+                                                      u8 *dig, QPos qp, u32 c, u64 a,
+                                                      const u32 *d_a, u32 none, ChainSeed *cs,
+                                                      u32 *tact, u32 *tent, u32 *maxr) {
+    __shared__ u32 red[NWAVES + 1];
+    const u64 base = (u64)blockIdx.x * CQ_TILE;
+    if (d_a) a = *d_a < a ? *d_a : a;
+    u32 nact = 0, ent = 0, mx = 0;
+    for (int rr = 0; rr < CQ_ROUNDS; rr++) {
+        const u64 k = base + (u64)rr * BLOCK + threadIdx.x;
+        u32 r = 0, x = 0;
+        u64 w = 0;
+        if (k < a) {
+            const u64 slot = qp(k);
+            x = SA[slot];
+            w = PW[slot];
+            if (!Win::known(w)) {
+                w = pr.window_any(x, aligned);
+                PW[slot] = w;
+            }
+            dig[k] = (u8)(Win::v(w) ? Win::b(w, 1) : none);
+            u32 sb;
+            r = seed_run(pr, x, w, c, &sb);
+        }
+        u32 tot;
+        const u32 ex = block_exclusive_sum(r ? 1u : 0u, red, &tot);  // includes barriers
+        if (r) cs[base + nact + ex] = ChainSeed{x, r, w};
+        nact += tot;
+        ent += r;
+        mx = r > mx ? r : mx;
+    }
+    u32 tot;
+    block_exclusive_sum(ent, red, &tot);
+    if (threadIdx.x == 0) {
+        tact[blockIdx.x] = nact;
+        tent[blockIdx.x] = tot;
+    }
+    // same-address atomics serialise in L2: only raise the maximum when this
+    // wave beats the value already stored
+    const u32 wm = wave_max(mx);
+    if (lane_id() == 0 && wm > __atomic_load_n(maxr, __ATOMIC_RELAXED)) atomicMax(maxr, wm);
+}
+
+// Chain entries of one tile's compacted seeds, seed by seed from the tile's
+// emission offset teo[t]: (depth - 1, entry, its predecessor byte, its
+// window).  d_a (optional): the seed count on the device; the queue then
+// holds nchain - *d_a chain entries.
+__global__ void __launch_bounds__(BLOCK) k_chain_emit(Pred pr, const ChainSeed *cs, const u32 *tact,
+                                                      const u32 *teo, u32 c, u32 none, u64 nchain,
+                                                      const u32 *d_a, u32 *ek, u32 *ev, u8 *eb,
+                                                      u64 *ew, u32 *err) {
+    __shared__ u32 red[NWAVES + 1];
+    const u64 base = (u64)blockIdx.x * CQ_TILE;
+    if (d_a) nchain = *d_a <= nchain ? nchain - *d_a : 0;
+    const u32 na = tact[blockIdx.x];
+    u64 run = teo[blockIdx.x];
+    for (u32 b0 = 0; b0 < na; b0 += BLOCK) {
+        const u32 i = b0 + threadIdx.x;
+        ChainSeed sd{0, 0, 0};
+        if (i < na) sd = cs[base + i];
+        u32 tot;
+        const u32 ex = block_exclusive_sum(sd.r, red, &tot);  // includes barriers
+        if (sd.r) {
+            const u64 o = run + ex;
+            if (o + sd.r > nchain) {
+                atomicOr(err, 2u);
+            } else {
+                u32 sb;
+                seed_run(pr, sd.x, sd.w, c, &sb);
+                const bool hs = sb >> 8;
+                for (u32 t = 1; t <= sd.r; t++) {
+                    ek[o + t - 1] = t - 1;
+                    ev[o + t - 1] = sd.x - t;
+                    eb[o + t - 1] = t < sd.r ? (u8)c : (u8)(hs ? sb & 255u : none);
+                    ew[o + t - 1] = chain_window(sd.w, sd.r, c, hs, sb & 255u, t);
+                }
+            }
+        }
+        run += tot;
+    }
+}
+
 inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *dig, QPos qp,
                          u32 c, const u32 *d_start, u64 limit, u32 *d_tmp, u32 none) {
     const u64 a = d.read_u32(d_start);
     u32 *err = d.err;
     if (a > limit) fail(PSK_EDEVICE, "queue seed count exceeds its bucket part");
+    if (a == 0) {
+        if (limit) fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
+        return;
+    }
     ArenaScope mk(*d.a);
-    u32 *run = d.a->get<u32>(a + 1);
-    unsigned short *stop = d.a->get<unsigned short>(a + 1);  // byte | has << 8
+    const u32 nt = ceil_div(a, CQ_TILE);
+    ChainSeed *cs = d.a->get<ChainSeed>(a);
+    u32 *tact = d.a->get<u32>(nt), *tent = d.a->get<u32>(nt), *teo = d.a->get<u32>(nt);
     u32 *maxr = d_tmp;
     d.zero(maxr, sizeof(u32));
     const u64 nchain = limit - a;
     {
-        int prec = d.pbegin("chain8_runs", 22.0 * a);
-        const u64 aa = a;
-        // one thread per seed; the grid is a multiple of the wave so every
-        // lane takes part in the wave-wide maximum (one atomic per wave)
-        u64 blocks = std::min<u64>((aa + BLOCK - 1) / BLOCK, 65536u * 4);
-        k_for_each_waves<<<(u32)std::max<u64>(blocks, 1), BLOCK, 0, d.s>>>(
-            [=] __device__(u64 k, bool live) {
-                u32 r = 0;
-                if (live) {
-                    const u64 slot = qp(k);
-                    u32 x = SA[slot];
-                    u64 w = PW[slot];
-                    if (!Win::known(w)) {
-                        w = pr.window_any(x, aligned);
-                        PW[slot] = w;
-                    }
-                    const u32 v = Win::v(w);
-                    const u32 cp = v ? Win::b(w, 1) : none;
-                    dig[k] = (u8)cp;
-                    u32 sb = 0;
-                    if (v && cp == c) {
-                        // the run of c's before x: inside the window, or
-                        // beyond it from the decided bits; sb = the byte
-                        // before the run (bit 8: there is one)
-                        r = 1;
-                        while (r < v && Win::b(w, r + 1) == c) r++;
-                        if (r < v) {
-                            sb = Win::b(w, r + 1) | 256u;
-                        } else if (!Win::e(w)) {
-                            r = pr.run_before(x);
-                            if (pr.has(x - r)) sb = pr.byte(x - r) | 256u;
-                        }
-                    }
-                    run[k] = r;
-                    stop[k] = (unsigned short)sb;
-                }
-                // same-address atomics serialise in L2: only raise the maximum
-                // when this wave beats the value already stored
-                u32 wm = wave_max(r);
-                if (lane_id() == 0 && wm > __atomic_load_n(maxr, __ATOMIC_RELAXED))
-                    atomicMax(maxr, wm);
-            },
-            aa);
+        int prec = d.pbegin("chain8_runs", 13.0 * a);
+        k_chain_runs<<<nt, BLOCK, 0, d.s>>>(pr, aligned, SA, PW, dig, qp, c, a, nullptr, none, cs,
+                                            tact, tent, maxr);
         PSK_LAUNCH_CHECK();
         d.pend(prec);
     }
-    if (nchain == 0) { mk.reset(); return; }
-    // emission offsets
-    u32 *eo = d.a->get<u32>(a + 1);
+    if (nchain == 0) return;
+    // emission offsets of the tiles
     u32 *tot = d_tmp + 1;
     scan_exclusive(
-        d, a, [=] __device__(u64 k) { return run[k]; },
-        [=] __device__(u64 k, u32 ex, u32) { eo[k] = ex; }, tot);
+        d, nt, [=] __device__(u64 t) { return tent[t]; },
+        [=] __device__(u64 t, u32 ex, u32) { teo[t] = ex; }, tot);
     u32 hv[2];
     d.read(hv, d_tmp, sizeof(hv));
     const u32 R = hv[0];
@@ -257,22 +336,9 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
     u32 *ek = d.a->get<u32>(nchain), *ev = d.a->get<u32>(nchain);
     u8 *eb = d.a->get<u8>(nchain);
     u64 *ew = d.a->get<u64>(nchain);
-    for_each(d, a, [=] __device__(u64 k) {
-        u32 r = run[k];
-        if (!r) return;
-        const u64 slot = qp(k);
-        u32 x = SA[slot], o = eo[k];
-        u64 w = PW[slot];
-        const u32 st = stop[k];
-        const bool hs = st >> 8;
-        const u32 sb = st & 255u;
-        for (u32 t = 1; t <= r; t++) {
-            ek[o + t - 1] = t - 1;
-            ev[o + t - 1] = x - t;
-            eb[o + t - 1] = t < r ? (u8)c : (u8)(hs ? sb : none);
-            ew[o + t - 1] = chain_window(w, r, c, hs, sb, t);
-        }
-    });
+    k_chain_emit<<<nt, BLOCK, 0, d.s>>>(pr, cs, tact, teo, c, none, nchain, nullptr, ek, ev, eb, ew,
+                                        err);
+    PSK_LAUNCH_CHECK();
     const u64 lim = limit, base = a;
     if (R <= 256) {
         multisplit(
@@ -311,7 +377,6 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
             dig[base + i] = eb[j];
         });
     }
-    mk.reset();
 }
 
 // The same queue fill without host synchronisation, for texts whose runs of
