@@ -596,7 +596,8 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     u32 *lmsbits = d.a->get<u32>(nwords + 1);
     u32 *ctr = d.a->get<u32>(2);
     d.zero(ctr, 2 * sizeof(u32));
-    k_lms32<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(sbits, lmsbits, nwords, ctr);
+    k_lms32<<<std::min<u32>(ceil_div(nwords, BLOCK), 2048), BLOCK, 0, d.s>>>(sbits, lmsbits, nwords,
+                                                                           ctr);
     PSK_LAUNCH_CHECK();
     // L item list (ascending positions); then the LMS and L counts in one read
     u32 *Litems = d.a->get<u32>(m);

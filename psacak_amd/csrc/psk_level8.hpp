@@ -148,8 +148,14 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // bucket statistics (Buckets::new psacak8.rs:355-374)
     u32 *stat = d.a->get<u32>(3 * 256 + 1);
     d.zero(stat, (3 * 256 + 1) * sizeof(u32));
-    k_lms_hist8<<<ceil_div(nwords, BLOCK), BLOCK, 0, d.s>>>(T, n, sbits, lmsbits, nwords, stat,
-                                                          stat + 256, stat + 512, stat + 768, SB);
+    {
+        // about 2048 workgroups, each of at most 2^15 words (2^20 positions)
+        u64 wpb = (nwords + 2047) / 2048;
+        wpb = std::min<u64>(std::max<u64>((wpb + BLOCK - 1) / BLOCK * BLOCK, BLOCK), 1u << 15);
+        k_lms_hist8<<<ceil_div(nwords, wpb), BLOCK, 0, d.s>>>(T, n, sbits, lmsbits, nwords, wpb,
+                                                             stat, stat + 256, stat + 512,
+                                                             stat + 768, SB);
+    }
     PSK_LAUNCH_CHECK();
     Buckets8 B;
     u32 host_stat[3 * 256 + 1];

@@ -331,8 +331,14 @@ __device__ __forceinline__ u32 lms_word(const u32 *sbits, u64 w) {
 constexpr u64 H_F = 21;  // bits per packed LDS counter field
 // startbits (batches): a text's first position has no predecessor in its
 // text, so it is never LMS.
-__global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, u64 nwords,
-                            u32 *cnt, u32 *nL, u32 *nLMS, u32 *m1, const u32 *startbits) {
+// A workgroup takes wpb consecutive words (wpb * 32 < 2^H_F positions, so
+// the packed LDS fields cannot overflow): few workgroups, so few global
+// atomics on the 3 x (distinct bytes) + 1 result counters, which every
+// workgroup adds to.
+__global__ void __launch_bounds__(BLOCK) k_lms_hist8(const u8 *t, u64 n, const u32 *sbits,
+                                                     u32 *lmsbits, u64 nwords, u64 wpb, u32 *cnt,
+                                                     u32 *nL, u32 *nLMS, u32 *m1,
+                                                     const u32 *startbits) {
     // four copies (lane & 3), padded onto different banks: a DNA text's
     // wave holds four byte values, whose lanes would otherwise serialise
     // on four counters
@@ -341,21 +347,36 @@ __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, 
     #pragma unroll
     for (int q = 0; q < 4; q++) h[q][threadIdx.x] = 0;
     __syncthreads();
-    u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x;
+    const u64 w0 = (u64)blockIdx.x * wpb;
+    const u64 w1 = w0 + wpb < nwords ? w0 + wpb : nwords;
+    const bool vec = (((uintptr_t)t) & 15u) == 0;
     u32 lcount = 0;
-    if (w < nwords) {
+    for (u64 w = w0 + threadIdx.x; w < w1; w += BLOCK) {
         u32 S = sbits[w];
         u32 lms = lms_word(sbits, w);
         if (startbits) lms &= ~startbits[w];
         lmsbits[w] = lms;
-        lcount = __popc(lms);
-        u64 p0 = w * 32;
-        u32 lim = (u32)((n - p0) < 32 ? (n - p0) : 32);
-        for (u32 b = 0; b < lim; b++) {
-            u32 c = t[p0 + b];
-            unsigned long long v = 1ull | ((unsigned long long)(~(S >> b) & 1u) << H_F) |
-                                   ((unsigned long long)((lms >> b) & 1u) << (2 * H_F));
-            atomicAdd(&h[lane_id() & 3][c], v);
+        lcount += __popc(lms);
+        const u64 p0 = w * 32;
+        const u32 lim = (u32)((n - p0) < 32 ? (n - p0) : 32);
+        u32 word[8];
+        if (vec && lim == 32) {
+            const uint4 a0 = *(const uint4 *)(t + p0), a1 = *(const uint4 *)(t + p0 + 16);
+            word[0] = a0.x; word[1] = a0.y; word[2] = a0.z; word[3] = a0.w;
+            word[4] = a1.x; word[5] = a1.y; word[6] = a1.z; word[7] = a1.w;
+        } else {
+            #pragma unroll
+            for (int q = 0; q < 8; q++) word[q] = 0;
+            for (u32 b = 0; b < lim; b++) word[b >> 2] |= (u32)t[p0 + b] << (8 * (b & 3));
+        }
+        #pragma unroll
+        for (u32 b = 0; b < 32; b++) {
+            if (b < lim) {
+                const u32 c = (word[b >> 2] >> (8 * (b & 3))) & 255u;
+                unsigned long long v = 1ull | ((unsigned long long)(~(S >> b) & 1u) << H_F) |
+                                       ((unsigned long long)((lms >> b) & 1u) << (2 * H_F));
+                atomicAdd(&h[lane_id() & 3][c], v);
+            }
         }
     }
     u32 tot;
@@ -372,15 +393,16 @@ __global__ void k_lms_hist8(const u8 *t, u64 n, const u32 *sbits, u32 *lmsbits, 
     }
 }
 
-// LMS bits for a u32 level; *m1 += number of LMS positions.
-__global__ void k_lms32(const u32 *sbits, u32 *lmsbits, u64 nwords, u32 *m1) {
+// LMS bits for a u32 level; *m1 += number of LMS positions.  Grid-stride
+// over the words, so few workgroups add to the one counter.
+__global__ void __launch_bounds__(BLOCK) k_lms32(const u32 *sbits, u32 *lmsbits, u64 nwords,
+                                                 u32 *m1) {
     __shared__ u32 red[NWAVES + 1];
-    u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x;
     u32 lcount = 0;
-    if (w < nwords) {
+    for (u64 w = (u64)blockIdx.x * BLOCK + threadIdx.x; w < nwords; w += (u64)gridDim.x * BLOCK) {
         u32 lms = lms_word(sbits, w);
         lmsbits[w] = lms;
-        lcount = __popc(lms);
+        lcount += __popc(lms);
     }
     u32 tot;
     block_exclusive_sum(lcount, red, &tot);
