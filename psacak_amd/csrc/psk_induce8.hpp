@@ -210,21 +210,30 @@ __device__ __forceinline__ u32 seed_run(const Pred &pr, u32 x, u64 w, u32 c, u32
 
 constexpr int CQ_ROUNDS = 16;
 constexpr u64 CQ_TILE = (u64)BLOCK * CQ_ROUNDS;  // queue seeds per tile
+// Chain entries of depth <= CQ_DIRECT go straight to their queue slots; the
+// rest (runs longer than CQ_DIRECT: repeats, low-complexity stretches) are
+// emitted and split by depth.  Per-seed ranks of the CQ_DIRECT depths share
+// one 64-bit block scan: 9-bit fields (<= 256 seeds per scan step).
+constexpr u32 CQ_DIRECT = 7;
 
 // Runs of one tile of seeds [t CQ_TILE, ...) of a queue with a seeds: every
 // seed's predecessor byte goes to dig (`none` without one), and the seeds
 // with a chain are compacted, in seed order, to the start of the tile's own
-// range of cs; tact[t] counts them and tent[t] their chain entries.
-// d_a (optional): the seed count on the device, taken up to a (the queue's
-// size) in place of a.
+// range of cs; tact[t] counts them.  The tile's entry counts go to the
+// count matrix cm (nt tiles per row): row d-1 (d <= CQ_DIRECT) holds the
+// seeds with r >= d, row CQ_DIRECT the entries deeper than CQ_DIRECT.
 __global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, const u32 *SA, u64 *PW,
-                                                      u8 *dig, QPos qp, u32 c, u64 a,
-                                                      const u32 *d_a, u32 none, ChainSeed *cs,
-                                                      u32 *tact, u32 *tent, u32 *maxr) {
+                                                      u8 *dig, QPos qp, u32 c, u64 a, u32 none,
+                                                      ChainSeed *cs, u32 *tact, u32 *cm, u32 nt,
+                                                      u32 *maxr) {
     __shared__ u32 red[NWAVES + 1];
+    __shared__ u32 sums[CQ_DIRECT + 1];
+    if (threadIdx.x <= CQ_DIRECT) sums[threadIdx.x] = 0;
     const u64 base = (u64)blockIdx.x * CQ_TILE;
-    if (d_a) a = *d_a < a ? *d_a : a;
-    u32 nact = 0, ent = 0, mx = 0;
+    u32 nact = 0, mx = 0, deep = 0;
+    u32 ge[CQ_DIRECT];
+    #pragma unroll
+    for (u32 q = 0; q < CQ_DIRECT; q++) ge[q] = 0;
     for (int rr = 0; rr < CQ_ROUNDS; rr++) {
         const u64 k = base + (u64)rr * BLOCK + threadIdx.x;
         u32 r = 0, x = 0;
@@ -245,57 +254,95 @@ __global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, con
         const u32 ex = block_exclusive_sum(r ? 1u : 0u, red, &tot);  // includes barriers
         if (r) cs[base + nact + ex] = ChainSeed{x, r, w};
         nact += tot;
-        ent += r;
+        #pragma unroll
+        for (u32 q = 0; q < CQ_DIRECT; q++) ge[q] += r > q ? 1u : 0u;
+        deep += r > CQ_DIRECT ? r - CQ_DIRECT : 0u;
         mx = r > mx ? r : mx;
     }
-    u32 tot;
-    block_exclusive_sum(ent, red, &tot);
-    if (threadIdx.x == 0) {
-        tact[blockIdx.x] = nact;
-        tent[blockIdx.x] = tot;
+    #pragma unroll
+    for (u32 q = 0; q <= CQ_DIRECT; q++) {
+        const u32 v = wave_inclusive_sum(q < CQ_DIRECT ? ge[q] : deep);
+        if (lane_id() == WAVE - 1 && v) atomicAdd(&sums[q], v);
     }
+    __syncthreads();
+    if (threadIdx.x <= CQ_DIRECT) cm[(u64)threadIdx.x * nt + blockIdx.x] = sums[threadIdx.x];
+    if (threadIdx.x == 0) tact[blockIdx.x] = nact;
     // same-address atomics serialise in L2: only raise the maximum when this
     // wave beats the value already stored
     const u32 wm = wave_max(mx);
     if (lane_id() == 0 && wm > __atomic_load_n(maxr, __ATOMIC_RELAXED)) atomicMax(maxr, wm);
 }
 
-// Chain entries of one tile's compacted seeds, seed by seed from the tile's
-// emission offset teo[t]: (depth - 1, entry, its predecessor byte, its
-// window).  d_a (optional): the seed count on the device; the queue then
-// holds nchain - *d_a chain entries.
+// Chain entries of one tile's compacted seeds.  co is the exclusive scan of
+// the count matrix: co[(d-1) nt + t] is the queue offset (past the a seeds)
+// of tile t's first depth-d entry, co[CQ_DIRECT nt + t] that of its first
+// deep entry, and deep entries start at d0.  An entry of depth <= CQ_DIRECT
+// is written to its slot (SA, PW, dig); a deeper one is emitted as (depth -
+// CQ_DIRECT - 1, entry, predecessor byte, window) at its index past d0, in
+// seed order, for the split by depth.
 __global__ void __launch_bounds__(BLOCK) k_chain_emit(Pred pr, const ChainSeed *cs, const u32 *tact,
-                                                      const u32 *teo, u32 c, u32 none, u64 nchain,
-                                                      const u32 *d_a, u32 *ek, u32 *ev, u8 *eb,
+                                                      const u32 *co, u32 nt, u64 d0, u32 c,
+                                                      u32 none, QPos qp, u64 a, u64 limit, u32 *SA,
+                                                      u64 *PW, u8 *dig, u32 *ek, u32 *ev, u8 *eb,
                                                       u64 *ew, u32 *err) {
+    __shared__ u64 red64[NWAVES + 1];
     __shared__ u32 red[NWAVES + 1];
-    const u64 base = (u64)blockIdx.x * CQ_TILE;
-    if (d_a) nchain = *d_a <= nchain ? nchain - *d_a : 0;
-    const u32 na = tact[blockIdx.x];
-    u64 run = teo[blockIdx.x];
+    const u32 t = blockIdx.x;
+    const u64 base = (u64)t * CQ_TILE;
+    const u32 na = tact[t];
+    u64 off[CQ_DIRECT];
+    #pragma unroll
+    for (u32 q = 0; q < CQ_DIRECT; q++) off[q] = co[(u64)q * nt + t];
+    u64 doff = co[(u64)CQ_DIRECT * nt + t];
+    const u64 nchain = limit - a;
     for (u32 b0 = 0; b0 < na; b0 += BLOCK) {
         const u32 i = b0 + threadIdx.x;
         ChainSeed sd{0, 0, 0};
         if (i < na) sd = cs[base + i];
-        u32 tot;
-        const u32 ex = block_exclusive_sum(sd.r, red, &tot);  // includes barriers
-        if (sd.r) {
-            const u64 o = run + ex;
-            if (o + sd.r > nchain) {
-                atomicOr(err, 2u);
-            } else {
-                u32 sb;
-                seed_run(pr, sd.x, sd.w, c, &sb);
-                const bool hs = sb >> 8;
-                for (u32 t = 1; t <= sd.r; t++) {
-                    ek[o + t - 1] = t - 1;
-                    ev[o + t - 1] = sd.x - t;
-                    eb[o + t - 1] = t < sd.r ? (u8)c : (u8)(hs ? sb & 255u : none);
-                    ew[o + t - 1] = chain_window(sd.w, sd.r, c, hs, sb & 255u, t);
+        const u32 r = sd.r;
+        u64 flags = 0;
+        #pragma unroll
+        for (u32 q = 0; q < CQ_DIRECT; q++) flags |= (u64)(r > q ? 1u : 0u) << (9 * q);
+        u64 tot;
+        const u64 ex = block_exclusive_sum(flags, red64, &tot);  // includes barriers
+        const u32 dr = r > CQ_DIRECT ? r - CQ_DIRECT : 0u;
+        u32 dtot;
+        const u32 dex = block_exclusive_sum(dr, red, &dtot);
+        if (r) {
+            u32 sb;
+            seed_run(pr, sd.x, sd.w, c, &sb);
+            const bool hs = sb >> 8;
+            const u32 last = hs ? sb & 255u : none;
+            const u32 rd = r < CQ_DIRECT ? r : CQ_DIRECT;
+            for (u32 q = 0; q < rd; q++) {
+                const u64 pos = off[q] + ((ex >> (9 * q)) & 511u);
+                if (pos >= d0 || a + pos >= limit) {
+                    atomicOr(err, 2u);
+                    continue;
+                }
+                const u64 slot = qp(a + pos);
+                SA[slot] = sd.x - (q + 1);
+                PW[slot] = chain_window(sd.w, r, c, hs, sb & 255u, q + 1);
+                dig[a + pos] = (u8)(q + 1 < r ? c : last);
+            }
+            if (dr) {
+                const u64 o = doff - d0 + dex;
+                if (doff < d0 || o + dr > nchain - d0) {
+                    atomicOr(err, 2u);
+                } else {
+                    for (u32 u = 0; u < dr; u++) {
+                        const u32 dep = CQ_DIRECT + 1 + u;
+                        ek[o + u] = u;
+                        ev[o + u] = sd.x - dep;
+                        eb[o + u] = dep < r ? (u8)c : (u8)last;
+                        ew[o + u] = chain_window(sd.w, r, c, hs, sb & 255u, dep);
+                    }
                 }
             }
         }
-        run += tot;
+        #pragma unroll
+        for (u32 q = 0; q < CQ_DIRECT; q++) off[q] += (tot >> (9 * q)) & 511u;
+        doff += dtot;
     }
 }
 
@@ -308,41 +355,61 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
         if (limit) fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
         return;
     }
+    (void)d_tmp;
     ArenaScope mk(*d.a);
     const u32 nt = ceil_div(a, CQ_TILE);
     ChainSeed *cs = d.a->get<ChainSeed>(a);
-    u32 *tact = d.a->get<u32>(nt), *tent = d.a->get<u32>(nt), *teo = d.a->get<u32>(nt);
-    u32 *maxr = d_tmp;
-    d.zero(maxr, sizeof(u32));
+    const u64 rows = (u64)(CQ_DIRECT + 1) * nt;
+    u32 *tact = d.a->get<u32>(nt), *cm = d.a->get<u32>(rows), *co = d.a->get<u32>(rows);
+    u32 *hc = d.a->get<u32>(4);  // largest run, entries in all, first deep entry
+    d.zero(hc, 4 * sizeof(u32));
     const u64 nchain = limit - a;
     {
         int prec = d.pbegin("chain8_runs", 13.0 * a);
-        k_chain_runs<<<nt, BLOCK, 0, d.s>>>(pr, aligned, SA, PW, dig, qp, c, a, nullptr, none, cs,
-                                            tact, tent, maxr);
+        k_chain_runs<<<nt, BLOCK, 0, d.s>>>(pr, aligned, SA, PW, dig, qp, c, a, none, cs, tact, cm,
+                                            nt, hc);
         PSK_LAUNCH_CHECK();
         d.pend(prec);
     }
     if (nchain == 0) return;
-    // emission offsets of the tiles
-    u32 *tot = d_tmp + 1;
+    // queue offsets of every (depth, tile) and of the deep entries
+    const u64 drow = (u64)CQ_DIRECT * nt;
     scan_exclusive(
-        d, nt, [=] __device__(u64 t) { return tent[t]; },
-        [=] __device__(u64 t, u32 ex, u32) { teo[t] = ex; }, tot);
-    u32 hv[2];
-    d.read(hv, d_tmp, sizeof(hv));
+        d, rows, [=] __device__(u64 i) { return cm[i]; },
+        [=] __device__(u64 i, u32 ex, u32) {
+            co[i] = ex;
+            if (i == drow) hc[2] = ex;
+        },
+        hc + 1);
+    u32 hv[3];
+    d.read(hv, hc, sizeof(hv));
     const u32 R = hv[0];
-    if ((u64)hv[1] != nchain) fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
-    // emit (depth-1, entry, its predecessor byte, its window) seed by seed
-    u32 *ek = d.a->get<u32>(nchain), *ev = d.a->get<u32>(nchain);
-    u8 *eb = d.a->get<u8>(nchain);
-    u64 *ew = d.a->get<u64>(nchain);
-    k_chain_emit<<<nt, BLOCK, 0, d.s>>>(pr, cs, tact, teo, c, none, nchain, nullptr, ek, ev, eb, ew,
-                                        err);
-    PSK_LAUNCH_CHECK();
-    const u64 lim = limit, base = a;
-    if (R <= 256) {
+    const u64 d0 = hv[2];
+    if ((u64)hv[1] != nchain || d0 > nchain)
+        fail(PSK_EDEVICE, "chain entries do not fill their bucket part");
+    const u64 ndeep = nchain - d0;
+    u32 *ek = nullptr, *ev = nullptr;
+    u8 *eb = nullptr;
+    u64 *ew = nullptr;
+    if (ndeep) {
+        ek = d.a->get<u32>(ndeep);
+        ev = d.a->get<u32>(ndeep);
+        eb = d.a->get<u8>(ndeep);
+        ew = d.a->get<u64>(ndeep);
+    }
+    {
+        int pe = d.pbegin("chain8_emit", 16.0 * a + 13.0 * (double)nchain);
+        k_chain_emit<<<nt, BLOCK, 0, d.s>>>(pr, cs, tact, co, nt, d0, c, none, qp, a, limit, SA, PW,
+                                            dig, ek, ev, eb, ew, err);
+        PSK_LAUNCH_CHECK();
+        d.pend(pe);
+    }
+    if (!ndeep) return;
+    // deep entries in breadth-first order: a stable split by depth
+    const u64 lim = limit, base = a + d0;
+    if (R - CQ_DIRECT <= 256) {
         multisplit(
-            d, nchain,
+            d, ndeep,
             [=] __device__(u64 j, u32 *key) {
                 *key = ek[j];
                 return true;
@@ -361,15 +428,15 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
             nullptr);
     } else {
         // long runs: stable LSD sort of (depth, emission index)
-        u64 *k0 = d.a->get<u64>(nchain), *k1 = d.a->get<u64>(nchain);
-        u32 *v0 = d.a->get<u32>(nchain), *v1 = d.a->get<u32>(nchain);
-        for_each(d, nchain, [=] __device__(u64 j) {
+        u64 *k0 = d.a->get<u64>(ndeep), *k1 = d.a->get<u64>(ndeep);
+        u32 *v0 = d.a->get<u32>(ndeep), *v1 = d.a->get<u32>(ndeep);
+        for_each(d, ndeep, [=] __device__(u64 j) {
             k0[j] = ek[j];
             v0[j] = (u32)j;
         });
-        bool sw = radix_sort_kv(d, nchain, k0, v0, k1, v1, bit_width(R - 1));
+        bool sw = radix_sort_kv(d, ndeep, k0, v0, k1, v1, bit_width(R - CQ_DIRECT - 1));
         const u32 *V = sw ? v1 : v0;
-        for_each(d, nchain, [=] __device__(u64 i) {
+        for_each(d, ndeep, [=] __device__(u64 i) {
             u32 j = V[i];
             const u64 slot = qp(base + i);
             SA[slot] = ev[j];
