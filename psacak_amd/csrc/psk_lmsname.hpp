@@ -410,7 +410,7 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
 inline u32 lms_slot_names(Dev &d, LmsRounds &st, u32 m1, u32 *tails) {
     u32 *tl = tails, *F = st.F;
     const u32 mm1 = m1;
-    scan_exclusive(
+    compact_exclusive(
         d, m1, [=] __device__(u64 j) { return F[j]; },
         [=] __device__(u64 j, u32 ex, u32 v) {
             if (v) {
@@ -745,7 +745,7 @@ inline bool name_lms_dict(Dev &d, PackedRanks rk, u64 sigma, u64 m, const u32 *l
         for_each(d, K1, [=] __device__(u64 j) {
             if (ugid[j] != ~0u) glab[ugid[j]] = start[j];
         });
-        scan_exclusive(
+        compact_exclusive(
             d, m1, [=] __device__(u64 r) { return ugid[SL[r]] != ~0u ? 1u : 0u; },
             [=] __device__(u64 r, u32 ex, u32 v) {
                 if (v) {

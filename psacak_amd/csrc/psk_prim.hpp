@@ -221,7 +221,11 @@ __global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
 // workgroup combines those of the tiles before it (few tiles: a read of at
 // most SCAN_FUSE_MAX words from L2 instead of a single-workgroup launch), and
 // the last one writes the grand aggregate to *total (if non-null).
-template <class Op, bool INCLUSIVE, bool FUSED, class In, class Out>
+// SKIP (sum scans for compactions, whose out acts only on items with v != 0):
+// a tile whose items sum to 0 makes no calls of out; unfused, tile_off then
+// holds nt + 1 entries (the last one the grand total, which block 0 copies
+// to *total).
+template <class Op, bool INCLUSIVE, bool FUSED, class In, class Out, bool SKIP = false>
 __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off,
                                                       u32 *total) {
     __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
@@ -234,6 +238,14 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, con
         block_incl<Op>(acc, lds, &tbase);  // includes barriers
     } else {
         tbase = tile_off[blockIdx.x];
+    }
+    if (SKIP) {
+        if (!FUSED && total && blockIdx.x == 0 && threadIdx.x == 0) *total = tile_off[gridDim.x];
+        const u32 own = FUSED ? tile_off[blockIdx.x] : tile_off[blockIdx.x + 1] - tbase;
+        if (own == 0) {
+            if (FUSED && total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = tbase;
+            return;
+        }
     }
     u32 v[SCAN_ROUNDS];
     #pragma unroll
@@ -302,6 +314,38 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
         k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, d_total);
         PSK_LAUNCH_CHECK();
         k_scan_apply<OpSum, false, false><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta, nullptr);
+    }
+    PSK_LAUNCH_CHECK();
+    mk.reset();
+}
+
+// Exclusive sum scan for a compaction: out is called only for the items of
+// tiles with a nonzero sum, so it must act only on items with v != 0 (a
+// sparse predicate then costs one read of the items, not two).
+template <class In, class Out>
+inline void compact_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
+    if (n == 0) {
+        if (d_total) d.zero(d_total, sizeof(u32));
+        return;
+    }
+    ArenaScope mk(*d.a);
+    u32 nt = ceil_div(n, SCAN_TILE);
+    u32 *ta = d.a->get<u32>((u64)nt + 1);
+    if (nt > 1) k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(in, n, ta);
+    PSK_LAUNCH_CHECK();
+    if (nt <= SCAN_FUSE_MAX) {
+        if (nt == 1) {
+            // no tile aggregate was written: the one tile is never skipped
+            k_scan_apply<OpSum, false, true><<<1, BLOCK, 0, d.s>>>(in, out, n, ta, d_total);
+        } else {
+            k_scan_apply<OpSum, false, true, In, Out, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta,
+                                                                                  d_total);
+        }
+    } else {
+        k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, ta + nt);
+        PSK_LAUNCH_CHECK();
+        k_scan_apply<OpSum, false, false, In, Out, true><<<nt, BLOCK, 0, d.s>>>(in, out, n, ta,
+                                                                               d_total);
     }
     PSK_LAUNCH_CHECK();
     mk.reset();
