@@ -144,7 +144,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     u32 *glab = d.a->get<u32>(gcap), *glab2 = d.a->get<u32>(gcap);
     u64 *keys = d.a->get<u64>(A);
     u64 *kt = d.a->get<u64>(A);
-    u32 *cnt = d.a->get<u32>(2);
+    u32 *cnt = d.a->get<u32>(4);  // after a round: items and groups left, large-group tiles and items
     const int B = bit_width(m);
     int Bh = B;
     const bool byrank = rank != nullptr && tails != nullptr && k > 0;
@@ -163,6 +163,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     u64 h = 1;
     int rounds = 0;
     u32 G = 0;             // tie groups entering the round (after round 1)
+    u32 NT = 0, LI = 0;    // ... the tiles and items of the large ones
     bool carried = false;  // the group ids hold run offsets
     bool skipped = false;  // a round left labels unwritten
     while (A > 0) {
@@ -224,7 +225,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         // place and sort it by the label bits alone
         bool sw = first ? radix_sort_kv(d, A, keys, act, kt, vt, Bh + (f1 ? B1 : B))
                         : group_sort_kv(d, A, G, gid, prevc ? RUN_GID_MASK : ~0u, keys, act, kt,
-                                        vt, B);
+                                        vt, B, NT, LI);
         const u64 *K = sw ? kt : keys;
         const u32 *V = sw ? vt : act;
         u32 *nxt = sw ? act : vt;  // the buffer not holding V
@@ -250,12 +251,12 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
                                  last_slot_labels, !cr && skipped};
             k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, sink, nxt,
                                                tg, ts, tk, te,
-                                               tq, lastconv, mm, gl, gid2, glab2, err);
+                                               tq, lastconv, mm, gl, gid2, glab2, err, cnt);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
             m3.reset();
         }
-        u32 ag[2];
+        u32 ag[4];
         d.read(ag, cnt, sizeof(ag));
         if (SA) d.padd(pr, 4.0 * (double)(AA - ag[0]));
         A = ag[0];
@@ -270,6 +271,8 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
         }
         Bh = bit_width(ag[1] ? ag[1] - 1 : 0);
         G = ag[1];
+        NT = ag[2];
+        LI = ag[3];
         carried = cr;
         skipped = skipped || cr;
         h *= 2;

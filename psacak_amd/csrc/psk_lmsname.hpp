@@ -317,6 +317,7 @@ struct LmsRounds {
     u64 off = 0;  // digits compared so far by every tied substring
     int Bg = 0;   // bits of the group ids
     u32 G = 0;    // tie groups (dense ids 0..G-1 in gid)
+    u32 NT = 0, LI = 0;  // tiles and items of the large ones (group_sort_kv)
     bool grouped = false;  // act holds the tie groups contiguously, in id order
     int rounds = 0;
 
@@ -331,7 +332,7 @@ struct LmsRounds {
         kt = d.a->get<u64>(m1);
         L = d.a->get<u32>(m1);
         F = d.a->get<u32>(m1);
-        cnt = d.a->get<u32>(4);
+        cnt = d.a->get<u32>(6);  // the relabel's four counts, then two flags
     }
 };
 
@@ -366,7 +367,7 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
         bool sw = !st.grouped
                       ? radix_sort_kv(d, st.A, st.keys, st.act, st.kt, st.vt, st.Bg + LB)
                       : group_sort_kv(d, st.A, st.G, st.gid, ~0u, st.keys, st.act, st.kt, st.vt,
-                                      LB);
+                                      LB, st.NT, st.LI);
         const u64 *K = sw ? st.kt : st.keys;
         const u32 *V = sw ? st.vt : st.act;
         u32 *nxt = sw ? st.act : st.vt;
@@ -384,12 +385,12 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
             int pr = d.pbegin("lms_name_relabel", 28.0 * st.A);
             k_rel_apply<<<nt, BLOCK, 0, d.s>>>(in, V, FinishSink{st.L, st.F}, nxt, tg, ts, tk, te,
                                                tq, false, (u64)m1, first ? nullptr : st.glab,
-                                               st.gid2, st.glab2, err);
+                                               st.gid2, st.glab2, err, st.cnt);
             PSK_LAUNCH_CHECK();
             d.pend(pr);
             m3.reset();
         }
-        u32 ag[2];
+        u32 ag[4];
         d.read(ag, st.cnt, sizeof(ag));
         const u64 before = st.A;
         st.A = ag[0];
@@ -399,6 +400,8 @@ inline bool lms_refine(Dev &d, LmsRounds &st, RF rk, u64 sigma, u64 m, const u32
         trace_round(d, "lms names", st.rounds, before, st.Bg + LB, st.gid, st.A, ~0u, ag[1]);
         st.Bg = bit_width(ag[1] ? ag[1] - 1 : 0);
         st.G = ag[1];
+        st.NT = ag[2];
+        st.LI = ag[3];
         st.grouped = true;  // the relabel emits the kept items by group
         st.off += (u64)D;
     }
@@ -457,7 +460,7 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
     for_each(d, m1, [=] __device__(u64 i) { act[i] = (u32)i; });
     st.A = m1;
     if (E.EB) {
-        E.flag = st.cnt + 2;  // cnt[0..1] belong to the rounds
+        E.flag = st.cnt + 4;  // cnt[0..3] belong to the rounds
         d.zero(E.flag, sizeof(u32));
     }
     if (!lms_refine(d, st, rk, sigma, m, lms_text, m1, D, max_rounds, E) ||

@@ -30,16 +30,23 @@ struct Dev {
     void padd(int r, double bytes) { if (prof) prof->add(r, bytes); }
 
     u64 syncs = 0;  // host synchronisations with the stream (readbacks, staged writes)
-    u32 read_u32(const u32 *d) {
+    bool trace = false;
+    // with trace: every host synchronisation prints the source line asking for it
+    void sync_site(const char *fn, int line) {
+        syncs++;
+        if (trace) fprintf(stderr, "[psk] sync %s:%d\n", fn, line);
+    }
+    u32 read_u32(const u32 *d, const char *fn = __builtin_FUNCTION(), int line = __builtin_LINE()) {
         PSK_HIP(hipMemcpyAsync(pinned, d, sizeof(u32), hipMemcpyDeviceToHost, s));
         PSK_HIP(hipStreamSynchronize(s));
-        syncs++;
+        sync_site(fn, line);
         return pinned[0];
     }
-    void read(void *host, const void *d, u64 bytes) {
+    void read(void *host, const void *d, u64 bytes, const char *fn = __builtin_FUNCTION(),
+              int line = __builtin_LINE()) {
         PSK_HIP(hipMemcpyAsync(pinned, d, bytes, hipMemcpyDeviceToHost, s));
         PSK_HIP(hipStreamSynchronize(s));
-        syncs++;
+        sync_site(fn, line);
         memcpy(host, pinned, bytes);
     }
     void write(void *d, const void *host, u64 bytes) {
@@ -59,9 +66,8 @@ struct Dev {
         if (bytes) PSK_HIP(hipMemsetAsync(d, 0, bytes, s));
     }
     void fill_u32(u32 *d, u32 v, u64 count);
-    // PSACAK_AMD_TRACE=1 in the environment: refinement rounds print their
-    // sizes to stderr (extra kernels and host syncs; diagnostics only)
-    bool trace = false;
+    // PSACAK_AMD_TRACE=1 in the environment (trace above): refinement rounds
+    // print their sizes to stderr (extra kernels and host syncs; diagnostics only)
     // with trace: synchronise and print the time since the previous mark
     double tmark = -1;
     void phase(const char *name) {
@@ -176,7 +182,7 @@ __global__ void k_scan_reduce(In in, u64 n, u32 *tile_agg) {
 // one workgroup: exclusive scan (Op) of nt tile aggregates in place; *total
 // (if non-null) receives the grand aggregate.
 template <class Op>
-__device__ void scan_tiles_wg(u32 *a, u32 nt, u32 *total) {
+__device__ u32 scan_tiles_wg(u32 *a, u32 nt, u32 *total) {
     __shared__ u32 lds[NWAVES + 1];
     u32 carry = Op::id();
     for (u64 base = 0; base < nt; base += (u64)BLOCK * TSCAN_IPT) {
@@ -206,6 +212,7 @@ __device__ void scan_tiles_wg(u32 *a, u32 nt, u32 *total) {
         carry = Op::op(carry, agg);
     }
     if (total && threadIdx.x == 0) *total = carry;
+    return carry;
 }
 
 template <class Op>
@@ -213,40 +220,15 @@ __global__ void k_scan_tiles(u32 *a, u32 nt, u32 *total) {
     scan_tiles_wg<Op>(a, nt, total);
 }
 
-// One tile of SCAN_TILE items per workgroup: coalesced loads into LDS, each
-// thread scans SCAN_ROUNDS consecutive items, one block scan of the thread
-// totals, and coalesced calls of out in index order.  (LDS rows are padded by
-// one word per 32 to spread the per-thread runs over the banks.)
-// FUSED: tile_off holds the tiles' aggregates, not their scan; each
-// workgroup combines those of the tiles before it (few tiles: a read of at
-// most SCAN_FUSE_MAX words from L2 instead of a single-workgroup launch), and
-// the last one writes the grand aggregate to *total (if non-null).
-// SKIP (sum scans for compactions, whose out acts only on items with v != 0):
-// a tile whose items sum to 0 makes no calls of out; unfused, tile_off then
-// holds nt + 1 entries (the last one the grand total, which block 0 copies
-// to *total).
-template <class Op, bool INCLUSIVE, bool FUSED, class In, class Out, bool SKIP = false>
-__global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off,
-                                                      u32 *total) {
-    __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
-    __shared__ u32 lds[NWAVES + 1];
-    const u64 base = (u64)blockIdx.x * SCAN_TILE;
-    u32 tbase;
-    if (FUSED) {
-        u32 acc = Op::id();
-        for (u32 i = threadIdx.x; i < blockIdx.x; i += BLOCK) acc = Op::op(acc, tile_off[i]);
-        block_incl<Op>(acc, lds, &tbase);  // includes barriers
-    } else {
-        tbase = tile_off[blockIdx.x];
-    }
-    if (SKIP) {
-        if (!FUSED && total && blockIdx.x == 0 && threadIdx.x == 0) *total = tile_off[gridDim.x];
-        const u32 own = FUSED ? tile_off[blockIdx.x] : tile_off[blockIdx.x + 1] - tbase;
-        if (own == 0) {
-            if (FUSED && total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = tbase;
-            return;
-        }
-    }
+// The items [base, base + SCAN_TILE) of a scan whose tiles before this one
+// combine to tbase: coalesced loads into LDS, each thread scans SCAN_ROUNDS
+// consecutive items, one block scan of the thread totals, and coalesced calls
+// of out in index order.  Returns the tile's aggregate.  buf holds
+// SCAN_TILE + SCAN_TILE / 32 words (rows padded by one word per 32 to spread
+// the per-thread runs over the banks), lds NWAVES + 1.
+template <class Op, bool INCLUSIVE, class In, class Out>
+__device__ __forceinline__ u32 scan_apply_tile(In in, Out out, u64 n, u64 base, u32 tbase,
+                                               u32 *buf, u32 *lds) {
     u32 v[SCAN_ROUNDS];
     #pragma unroll
     for (int r = 0; r < SCAN_ROUNDS; r++) {
@@ -287,6 +269,41 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, con
         const u32 j = (u32)r * BLOCK + threadIdx.x;
         if (i < n) out(i, buf[j + (j >> 5)], v[r]);
     }
+    return agg;
+}
+
+// One tile of SCAN_TILE items per workgroup (scan_apply_tile).
+// FUSED: tile_off holds the tiles' aggregates, not their scan; each
+// workgroup combines those of the tiles before it (few tiles: a read of at
+// most SCAN_FUSE_MAX words from L2 instead of a single-workgroup launch), and
+// the last one writes the grand aggregate to *total (if non-null).
+// SKIP (sum scans for compactions, whose out acts only on items with v != 0):
+// a tile whose items sum to 0 makes no calls of out; unfused, tile_off then
+// holds nt + 1 entries (the last one the grand total, which block 0 copies
+// to *total).
+template <class Op, bool INCLUSIVE, bool FUSED, class In, class Out, bool SKIP = false>
+__global__ void __launch_bounds__(BLOCK) k_scan_apply(In in, Out out, u64 n, const u32 *tile_off,
+                                                      u32 *total) {
+    __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
+    __shared__ u32 lds[NWAVES + 1];
+    const u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 tbase;
+    if (FUSED) {
+        u32 acc = Op::id();
+        for (u32 i = threadIdx.x; i < blockIdx.x; i += BLOCK) acc = Op::op(acc, tile_off[i]);
+        block_incl<Op>(acc, lds, &tbase);  // includes barriers
+    } else {
+        tbase = tile_off[blockIdx.x];
+    }
+    if (SKIP) {
+        if (!FUSED && total && blockIdx.x == 0 && threadIdx.x == 0) *total = tile_off[gridDim.x];
+        const u32 own = FUSED ? tile_off[blockIdx.x] : tile_off[blockIdx.x + 1] - tbase;
+        if (own == 0) {
+            if (FUSED && total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = tbase;
+            return;
+        }
+    }
+    const u32 agg = scan_apply_tile<Op, INCLUSIVE>(in, out, n, base, tbase, buf, lds);
     if (FUSED && total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = Op::op(tbase, agg);
 }
 
@@ -390,6 +407,99 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
     u32 q = nt / NXCD, r = nt % NXCD;
     u32 x = b % NXCD, i = b / NXCD;
     return x * q + (x < r ? x : r) + i;
+}
+
+// ---------------------------------------------------------------------------
+// Device-sized launches.  A count that only the device knows (the items still
+// tied after a sorting round, the tiles of a round's large groups, ...) is
+// read by the kernels themselves, so the host needs no readback to size the
+// launch: the grid comes from an upper bound the host does know, capped at
+// DN_GRID workgroups, and the workgroups stride over the tiles below the
+// device count.  The cap is a multiple of NXCD, so every tile index a block
+// strides over is congruent to the block index mod NXCD and xcd_tile keeps
+// each XCD on one contiguous range of tiles.  A device count of 0 costs a
+// launch of empty workgroups (a few microseconds).
+// ---------------------------------------------------------------------------
+#ifndef PSK_DN_GRID
+#define PSK_DN_GRID 2048
+#endif
+constexpr u32 DN_GRID = PSK_DN_GRID;  // 2048: 8 workgroups per CU
+inline u32 dn_grid(u64 tiles) {
+    u64 g = tiles < DN_GRID ? tiles : DN_GRID;
+    g = (g + NXCD - 1) / NXCD * NXCD;
+    return g ? (u32)g : NXCD;
+}
+
+template <typename F>
+__global__ void k_for_each_dn(F f, const u32 *dn) {
+    const u64 n = *dn;
+    const u64 stride = (u64)gridDim.x * BLOCK;
+    for (u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) f(i);
+}
+
+// f(i) for i in [0, *dn), with *dn <= ub
+template <typename F>
+inline void for_each_dn(Dev &d, u64 ub, const u32 *dn, F f) {
+    if (ub == 0) return;
+    k_for_each_dn<<<dn_grid(ceil_div(ub, BLOCK)), BLOCK, 0, d.s>>>(f, dn);
+    PSK_LAUNCH_CHECK();
+}
+
+template <class In>
+__global__ void __launch_bounds__(BLOCK) k_scan_reduce_dn(In in, const u32 *dn, u32 *tile_agg) {
+    __shared__ u32 lds[NWAVES + 1];
+    const u64 n = *dn;
+    const u64 nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    for (u64 t = blockIdx.x; t < nt; t += gridDim.x) {
+        const u64 base = t * SCAN_TILE;
+        u32 acc = 0;
+        for (int r = 0; r < SCAN_ROUNDS; r++) {
+            const u64 i = base + (u64)r * BLOCK + threadIdx.x;
+            if (i < n) acc += in(i);
+        }
+        u32 agg;
+        block_incl<OpSum>(acc, lds, &agg);  // ends with a barrier: lds is free again
+        if (threadIdx.x == 0) tile_agg[t] = agg;
+    }
+}
+
+// tail (optional): tail[*dn] receives the total as well (a scanned count
+// matrix whose last entry's count is read as the difference to the next)
+__global__ void k_scan_tiles_dn(u32 *a, const u32 *dn, u32 *total, u32 *tail) {
+    const u64 n = *dn;
+    const u32 c = scan_tiles_wg<OpSum>(a, (u32)((n + SCAN_TILE - 1) / SCAN_TILE), total);
+    if (tail && threadIdx.x == 0) tail[n] = c;
+}
+
+template <class In, class Out>
+__global__ void __launch_bounds__(BLOCK) k_scan_apply_dn(In in, Out out, const u32 *dn,
+                                                         const u32 *tile_off) {
+    __shared__ u32 buf[SCAN_TILE + SCAN_TILE / 32];
+    __shared__ u32 lds[NWAVES + 1];
+    const u64 n = *dn;
+    const u64 nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    for (u64 t = blockIdx.x; t < nt; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();  // the previous tile's reads of buf are done
+        scan_apply_tile<OpSum, false>(in, out, n, t * SCAN_TILE, tile_off[t], buf, lds);
+    }
+}
+
+// Exclusive sum scan over [0, *dn), *dn <= ub: as scan_exclusive, with d_total
+// (may be null) and tail (may be null, see k_scan_tiles_dn) receiving the sum.
+template <class In, class Out>
+inline void scan_exclusive_dn(Dev &d, u64 ub, const u32 *dn, In in, Out out, u32 *d_total,
+                              u32 *tail = nullptr) {
+    ArenaScope mk(*d.a);
+    const u64 ntb = std::max<u64>(ceil_div(ub, SCAN_TILE), 1);
+    u32 *ta = d.a->get<u32>(ntb);
+    const u32 g = dn_grid(ntb);
+    k_scan_reduce_dn<<<g, BLOCK, 0, d.s>>>(in, dn, ta);
+    PSK_LAUNCH_CHECK();
+    k_scan_tiles_dn<<<1, BLOCK, 0, d.s>>>(ta, dn, d_total, tail);
+    PSK_LAUNCH_CHECK();
+    k_scan_apply_dn<<<g, BLOCK, 0, d.s>>>(in, out, dn, ta);
+    PSK_LAUNCH_CHECK();
+    mk.reset();
 }
 
 #ifndef PSK_MS_ROUNDS
@@ -1052,15 +1162,14 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
 
 // Group sort of (k, v)[0..A) with G groups whose ids are gid[i] & gmask
 // (dense, non-decreasing, 0 <= id < G): each group sorted stably by the key
-// bits [0, lob).  Returns true if the result is in (kt, vt).
+// bits [0, lob).  NT and LI: the tiles (RS_TILE items) and the items of the
+// groups of more than GS_LOCAL items, as the relabel that made the groups
+// counted them.  Returns true if the result is in (kt, vt).
 inline bool group_sort_kv(Dev &d, u64 A, u32 G, const u32 *gid, u32 gmask, u64 *k, u32 *v,
-                          u64 *kt, u32 *vt, int lob) {
+                          u64 *kt, u32 *vt, int lob, u32 NT, u32 LI) {
     if (A == 0 || G == 0) return false;
     ArenaScope mk(*d.a);
     u32 *start = d.a->get<u32>((u64)G + 1);
-    u32 *tb = d.a->get<u32>((u64)G + 1);
-    u32 *bias = d.a->get<u32>((u64)G + 1);
-    u32 *ctr = d.a->get<u32>(2);
     const u64 AA = A;
     const u32 GG = G;
     for_each(d, AA, [=] __device__(u64 i) {
@@ -1068,29 +1177,33 @@ inline bool group_sort_kv(Dev &d, u64 A, u32 G, const u32 *gid, u32 gmask, u64 *
         if (i == 0 || (gid[i - 1] & gmask) != g) start[g] = (u32)i;
         if (i + 1 == AA) start[GG] = (u32)AA;
     });
-    // tiles of the large groups, and each large group's shift from its place
-    // among the large groups' items to its place in the array
-    scan_exclusive(
-        d, GG,
-        [=] __device__(u64 g) {
-            const u32 s = start[g + 1] - start[g];
-            return s > GS_LOCAL ? (u32)((s + RS_TILE - 1) / RS_TILE) : 0u;
-        },
-        [=] __device__(u64 g, u32 ex, u32 v) {
-            tb[g] = ex;
-            if (g + 1 == GG) tb[GG] = ex + v;
-        },
-        ctr);
-    scan_exclusive(
-        d, GG,
-        [=] __device__(u64 g) {
-            const u32 s = start[g + 1] - start[g];
-            return s > GS_LOCAL ? s : 0u;
-        },
-        [=] __device__(u64 g, u32 ex, u32) { bias[g] = start[g] - ex; }, ctr + 1);
-    u32 hc[2];
-    d.read(hc, ctr, sizeof(hc));
-    const u32 NT = hc[0];
+    u32 *tb = nullptr, *bias = nullptr;
+    if (NT) {
+        // tiles of the large groups, and each large group's shift from its
+        // place among the large groups' items to its place in the array
+        tb = d.a->get<u32>((u64)G + 1);
+        bias = d.a->get<u32>((u64)G + 1);
+        u32 *tbl = tb, *bs = bias;
+        scan_exclusive(
+            d, GG,
+            [=] __device__(u64 g) {
+                const u32 s = start[g + 1] - start[g];
+                return s > GS_LOCAL ? (u32)((s + RS_TILE - 1) / RS_TILE) : 0u;
+            },
+            [=] __device__(u64 g, u32 ex, u32 v) {
+                tbl[g] = ex;
+                if (g + 1 == GG) tbl[GG] = ex + v;
+            },
+            nullptr);
+        scan_exclusive(
+            d, GG,
+            [=] __device__(u64 g) {
+                const u32 s = start[g + 1] - start[g];
+                return s > GS_LOCAL ? s : 0u;
+            },
+            [=] __device__(u64 g, u32 ex, u32) { bs[g] = start[g] - ex; }, nullptr);
+    }
+    const u32 hc[2] = {NT, LI};
     const int passes = NT ? (lob + 7) / 8 : 0;
     const bool swapped = (passes & 1) != 0;
     const u64 lomask = lob >= 64 ? ~0ull : ((1ull << lob) - 1);

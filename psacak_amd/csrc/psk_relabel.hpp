@@ -30,6 +30,12 @@ struct RelIn {
     bool solo = false;
     // flags of item i: group start, subgroup start, kept, group end
     __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge) const {
+        bool ns;
+        flags(i, gs, ss, kp, ge, ns);
+    }
+    // and ns: item i + 1 is in i's subgroup
+    __device__ __forceinline__ void flags(u64 i, bool &gs, bool &ss, bool &kp, bool &ge,
+                                          bool &ns) const {
         u64 k = K[i] & kmask, hi = k >> B;
         u64 kp_ = i ? K[i - 1] & kmask : ~0ull;
         u64 kn_ = i + 1 < A ? K[i + 1] & kmask : ~0ull;
@@ -37,7 +43,8 @@ struct RelIn {
         gs = i == 0 || (kp_ >> B) != hi;
         ss = i == 0 || kp_ != k || so;
         ge = i + 1 == A || (kn_ >> B) != hi;
-        kp = (!ss || (i + 1 < A && kn_ == k)) && !(tbit && (k & 1u)) && !so;
+        ns = i + 1 < A && kn_ == k && !so;
+        kp = (!ss || ns) && !(tbit && (k & 1u)) && !so;
     }
 };
 
@@ -245,10 +252,14 @@ __device__ void rmin_tiles_wg(u32 *a, u32 nt) {
 // exclusive max of group starts (tg) and of subgroup starts (ts), exclusive
 // sums of kept items (tk, total to cnt[0]) and of kept subgroups (tq, total
 // to cnt[1]), and, with last-slot labels, the exclusive suffix minimum of
-// group ends (te).  Grid: 4 blocks, or 5 with te.
+// group ends (te).  Grid: 4 blocks, or 5 with te.  cnt[2..3] (the next
+// round's large groups, added up by the apply) are cleared.
 __global__ void k_rel_tiles(u32 *tg, u32 *ts, u32 *tk, u32 *tq, u32 *te, u32 nt, u32 *cnt) {
     switch (blockIdx.x) {
-        case 0: scan_tiles_wg<OpMax>(tg, nt, nullptr); break;
+        case 0:
+            scan_tiles_wg<OpMax>(tg, nt, nullptr);
+            if (threadIdx.x == 0) cnt[2] = cnt[3] = 0;
+            break;
         case 1: scan_tiles_wg<OpMax>(ts, nt, nullptr); break;
         case 2: scan_tiles_wg<OpSum>(tk, nt, cnt); break;
         case 3: scan_tiles_wg<OpSum>(tq, nt, cnt + 1); break;
@@ -260,12 +271,15 @@ __global__ void k_rel_tiles(u32 *tg, u32 *ts, u32 *tk, u32 *tq, u32 *te, u32 nt,
 // label itself (glab == nullptr) or a dense group id whose label is glab[hi].
 // Kept items (not yet alone in their subgroup) go to nxt in sorted order, with
 // the dense id of their subgroup among the kept ones in gnx and that
-// subgroup's label in glnx[id].
+// subgroup's label in glnx[id].  The kept subgroups of more than GS_LOCAL
+// items (the next round's large groups, which the group sort takes through
+// LSD passes in RS_TILE-item tiles) add their tiles to cnt[2] and their items
+// to cnt[3], so the next round is sized by the one readback of cnt.
 template <class Sink>
 __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u32 *tg,
                             const u32 *ts, const u32 *tk, const u32 *te, const u32 *tq,
                             bool lastconv, u64 mm, const u32 *glab, u32 *gnx, u32 *glnx,
-                            u32 *err) {
+                            u32 *err, u32 *cnt) {
     __shared__ u32 l4x[4][NWAVES];
     __shared__ u32 wt[NWAVES];
     __shared__ u32 g1sh[REL_TILE];
@@ -303,8 +317,8 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
     u32 rg = tg[blockIdx.x], rs = ts[blockIdx.x], rk = tk[blockIdx.x], rq = tq[blockIdx.x];
     for (int r = 0; r < REL_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
-        bool gs = false, ss = false, kp = false, ge = false;
-        if (i < in.A) in.flags(i, gs, ss, kp, ge);
+        bool gs = false, ss = false, kp = false, ge = false, ns = false;
+        if (i < in.A) in.flags(i, gs, ss, kp, ge, ns);
         u32 g0 = gs ? (u32)i : 0u, s0 = ss ? (u32)i : 0u;
         u32 kk = kp ? 1u : 0u, qq = (kp && ss) ? 1u : 0u;
         u32 ag4[4];
@@ -329,6 +343,23 @@ __global__ void k_rel_apply(RelIn in, const u32 *V, Sink sink, u32 *nxt, const u
                     nxt[kk - 1] = p;
                     gnx[kk - 1] = sink.gid(qq - 1, kraw);
                     if (ss) glnx[qq - 1] = nl;
+                }
+            }
+        }
+        {
+            // the last item of a kept subgroup knows its size
+            const u32 sz = (kp && !ns) ? (u32)(i + 1 - s0) : 0u;
+            const bool big = sz > GS_LOCAL;
+            if (ballot(big)) {
+                u32 tl = big ? (u32)((sz + RS_TILE - 1) / RS_TILE) : 0u, it = big ? sz : 0u;
+                #pragma unroll
+                for (int o = WAVE / 2; o > 0; o >>= 1) {
+                    tl += __shfl_xor(tl, o, WAVE);
+                    it += __shfl_xor(it, o, WAVE);
+                }
+                if (lane_id() == 0) {
+                    atomicAdd(cnt + 2, tl);
+                    atomicAdd(cnt + 3, it);
                 }
             }
         }
