@@ -87,6 +87,17 @@ int psk_ctx_set_dict_naming(psk_ctx *ctx, int on);
  * levels below it.  0 = always recurse.  Default 128.  PSK_EINVAL outside
  * [0, 1024]. */
 int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024);
+/* Final induce of a byte text (level 0) with at least min_distinct_bytes
+ * distinct bytes: as two chain sorts over its L and S positions, the way the
+ * integer levels induce (cost independent of the alphabet), instead of one
+ * multi-split step per bucket and scan (the crate's bucket-by-bucket scans,
+ * psacak8.rs:127-190, 201-344).  Taken only when its workspace stays within
+ * psk_workspace_bytes8.  0 = always bucket steps.  Default 96 (measured
+ * crossover on 256 MiB random texts: bucket steps faster up to 64 distinct
+ * bytes, chain sorts from 128).  No
+ * counterpart in the crate; results are identical.  PSK_EINVAL outside
+ * [0, 256]. */
+int psk_ctx_set_chain_induce(psk_ctx *ctx, int min_distinct_bytes);
 /* Device bytes a psacak8 call on a text of n bytes needs beyond its
  * input and output buffers (the admission footprint). */
 uint64_t psk_workspace_bytes8(uint64_t n);

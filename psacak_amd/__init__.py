@@ -73,6 +73,7 @@ def _load():
         "psk_ctx_set_direct_rounds": ([vp, i32], i32),
         "psk_ctx_set_dict_naming": ([vp, i32], i32),
         "psk_ctx_set_doubling": ([vp, i32], i32),
+        "psk_ctx_set_chain_induce": ([vp, i32], i32),
         "psk_workspace_bytes8": ([u64], u64),
         "psk_workspace_bytes32": ([u64], u64),
         "psk_psacak_inplace": ([vp, vp, u64, vp, u64], i32),
@@ -202,6 +203,12 @@ class Context:
         distinct symbols are sorted by prefix doubling (0 = always recurse).
         Results are identical either way."""
         self._check(_lib.psk_ctx_set_doubling(self._h, int(distinct_per_1024)))
+
+    def set_chain_induce(self, min_distinct_bytes: int):
+        """Byte texts with at least this many distinct bytes take their final
+        induce as chain sorts instead of bucket steps (0 = always bucket
+        steps).  Results are identical either way."""
+        self._check(_lib.psk_ctx_set_chain_induce(self._h, int(min_distinct_bytes)))
 
     def set_dict_naming(self, on: bool):
         """Round 1 of direct naming at level 0 through a dictionary of the

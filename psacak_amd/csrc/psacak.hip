@@ -319,6 +319,12 @@ int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024) {
     return PSK_OK;
 }
 
+int psk_ctx_set_chain_induce(psk_ctx *ctx, int min_distinct_bytes) {
+    if (!ctx || min_distinct_bytes < 0 || min_distinct_bytes > 256) return PSK_EINVAL;
+    ctx->rt.chain_sigma = (u32)min_distinct_bytes;
+    return PSK_OK;
+}
+
 int psk_ctx_set_dict_naming(psk_ctx *ctx, int on) {
     if (!ctx || on < 0 || on > 1) return PSK_EINVAL;
     ctx->rt.dict_naming = on != 0;
@@ -342,34 +348,9 @@ int psk_ctx_workspace(const psk_ctx *ctx, uint64_t *reserved, uint64_t *peak) {
     return PSK_OK;
 }
 
-// Workspace bounds: level 0 holds type/LMS bit vectors (n/4), the digit
-// scratch (n), LMS lists and rank table (<= 4 x 2n), the placement records
-// (16 bytes per LMS suffix, <= 8n) unless the context is lean, plus the
-// recursion on <= n/2 symbols; each u32 level needs <= ~36 bytes per symbol.
-// Bounded by the worst case m1 = n/2 at every level (measured peaks: 31-37
-// bytes per input byte lean, 36-42 with records).
-static uint64_t levels_bytes(uint64_t n) {
-    uint64_t total = 0, m = n;
-    while (m > 3) {
-        total += m * 36 + (64ull << 20);
-        m /= 2;
-    }
-    return total + (64ull << 20);
-}
+uint64_t psk_workspace_bytes32(uint64_t n) { return psk::workspace_bytes32(n); }
 
-uint64_t psk_workspace_bytes32(uint64_t n) {
-    // the translation of an arbitrary text into bucket-pointer form at the
-    // first level adds about 4 bytes per symbol (measured: 81.7 bytes per
-    // symbol for psacak32_raw on 2^28 Zipf symbols)
-    return levels_bytes(n) + 4 * n;
-}
-
-static uint64_t lean_bytes8(uint64_t n) {
-    // bit vectors, digit cache, byte windows (8n), LMS lists and names
-    return n / 4 + n + 17 * n + levels_bytes(n / 2) + (64ull << 20);
-}
-
-uint64_t psk_workspace_bytes8(uint64_t n) { return lean_bytes8(n) + 8 * n; }
+uint64_t psk_workspace_bytes8(uint64_t n) { return psk::workspace_bytes8(n); }
 
 int psk_psacak_inplace(psk_ctx *ctx, const uint8_t *text, uint64_t n, uint32_t *sa,
                        uint64_t sa_len) {
@@ -677,7 +658,7 @@ int psk_check32_dev(psk_ctx *ctx, const uint32_t *d_text, uint64_t n, const uint
 uint64_t psk_server_footprint(uint64_t bytes, int batched) {
     // text + SA + level workspace (the workers' contexts are lean), plus the
     // text-id partition of a batch
-    return lean_bytes8(bytes) + 5 * bytes + (batched ? 24 * bytes : 0);
+    return psk::lean_bytes8(bytes) + 5 * bytes + (batched ? 24 * bytes : 0);
 }
 
 // Worker loop.  psk_ctx_trim frees device memory, and hipFree waits for the

@@ -63,12 +63,45 @@ struct Runtime {
     // round 1 of direct naming at level 0 through a dictionary of the
     // distinct keys (false: by sorting every key)
     bool dict_naming = true;
+    // a byte text with at least this many distinct bytes takes its level-0
+    // final induce as chain sorts (final_induce32) instead of one multi-split
+    // step per bucket and scan; 0 = always bucket steps
+    u32 chain_sigma = 96;
     // level 0 places its sorted LMS suffixes from 16-byte (position, window)
     // records gathered by reduced index (one random read each, 16 bytes per
     // LMS suffix of extra workspace), or, when false, by gathering the
     // position and then the window (two random reads, no extra workspace)
     bool lms_records = true;
 };
+
+// Workspace bounds: level 0 holds type/LMS bit vectors (n/4), the digit
+// scratch (n), LMS lists and rank table (<= 4 x 2n), the placement records
+// (16 bytes per LMS suffix, <= 8n) unless the context is lean, plus the
+// recursion on <= n/2 symbols; each u32 level needs <= ~36 bytes per symbol.
+// Bounded by the worst case m1 = n/2 at every level (measured peaks: 31-37
+// bytes per input byte lean, 36-42 with records).
+inline u64 levels_bytes(u64 n) {
+    u64 total = 0, m = n;
+    while (m > 3) {
+        total += m * 36 + (64ull << 20);
+        m /= 2;
+    }
+    return total + (64ull << 20);
+}
+
+inline u64 workspace_bytes32(u64 n) {
+    // the translation of an arbitrary text into bucket-pointer form at the
+    // first level adds about 4 bytes per symbol (measured: 81.7 bytes per
+    // symbol for psacak32_raw on 2^28 Zipf symbols)
+    return levels_bytes(n) + 4 * n;
+}
+
+inline u64 lean_bytes8(u64 n) {
+    // bit vectors, digit cache, byte windows (8n), LMS lists and names
+    return n / 4 + n + 17 * n + levels_bytes(n / 2) + (64ull << 20);
+}
+
+inline u64 workspace_bytes8(u64 n) { return lean_bytes8(n) + 8 * n; }
 
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.
 template <class WF>
@@ -566,6 +599,101 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const
     return true;
 }
 
+// Final induce of a level from its sorted LMS suffixes (psacak32.rs:103-121,
+// then the L-scan and the S-scan as chain sorts): s[0..m) in bucket-pointer
+// form with its S-type bits, rank/tails/k its dense symbol ranks (rank may be
+// null: no short round-1 keys), lms_text its m1 LMS positions in text order,
+// SA1 their sorted reduced indices, Litems/Sitems its L and S positions in
+// ascending order.  RK and cnt: scratch of m + 1 words.  Writes all of SA.
+inline void final_induce32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, const u32 *rank,
+                           const u32 *tails, u32 k, const u32 *lms_text, u32 m1, const u32 *SA1,
+                           const u32 *Litems, u32 nL, const u32 *Sitems, u32 nS, u32 *RK,
+                           u32 *cnt, u32 *SA) {
+    Dev &d = rt.d;
+    ArenaScope mk(*d.a);
+    labels_L(d, s, m, sbits, RK);
+    // The final L sort's round 1 takes short keys (chain_sort, final) when
+    // the ranks leave room for a run offset beside them: its items are the
+    // predecessors of the sorted LMS suffixes in that order, then the L items
+    // whose successor is L or the sentinel.  cnt is free once the LMS labels
+    // are placed and holds that item order.
+    const bool shortL = rank && m1 && (u64)k < (1ull << 27);
+    u32 *ordL = cnt;
+    u64 *kpre = shortL ? d.a->get<u64>(m1) : nullptr;
+    const int B1 = bit_width(k) + 2;
+    // sorted LMS suffixes are grouped by first symbol already
+    if (m1) {
+        // SA1 holds reduced indices: one record per LMS position in text order
+        // (sequential reads), then one record gather per sorted entry gives
+        // the list and its symbols.  Records are (symbol, position), or with
+        // the short L keys (position, symbol, rank of the predecessor, rank + 1
+        // | the predecessor's L-run offset << 27)
+        ArenaScope m2(*d.a);
+        u64 *vals = d.a->get<u64>(m1);
+        u32 *list = d.a->get<u32>(m1 + 1);
+        u32 *err = d.err;
+        const u32 mm1 = m1;
+        if (shortL) {
+            uint4 *R = d.a->get<uint4>(m1);
+            const u32 *sb = sbits;
+            for_each(d, m1, [=] __device__(u64 p) {
+                const u32 x = lms_text[p];  // an LMS position: x >= 1, x-1 L-type
+                const u32 off = run_offset(sb, x - 1, false);
+                R[p] = make_uint4(x, s[x], rank[x - 1], (rank[x] + 1) | (off << 27));
+            });
+            const int b1 = B1;
+            for_each(d, m1, [=] __device__(u64 j) {
+                const u32 p = SA1[j];
+                uint4 r = make_uint4(0, 0, 0, 0);
+                if (p < mm1) r = R[p];
+                else atomicOr(err, 512u);
+                list[j] = r.x;
+                vals[j] = r.y;
+                kpre[j] = ((u64)r.z << b1) | ((u64)(r.w & ((1u << 27) - 1)) << 2) | 3ull |
+                          ((u64)(r.w >> 27) << RUN_KEY_SHIFT);
+            });
+        } else {
+            u64 *R = d.a->get<u64>(m1);
+            for_each(d, m1, [=] __device__(u64 p) {
+                const u32 x = lms_text[p];
+                R[p] = ((u64)s[x] << 32) | x;
+            });
+            for_each(d, m1, [=] __device__(u64 j) {
+                const u32 p = SA1[j];
+                u64 r = 0;
+                if (p < mm1) r = R[p];
+                else atomicOr(err, 512u);
+                list[j] = (u32)r;
+                vals[j] = r >> 32;
+            });
+        }
+        labels_lms_at_tails(d, s, list, vals, m1, RK, cnt, kpre);
+        if (shortL) for_each(d, m1, [=] __device__(u64 j) { ordL[j] = list[j] - 1; });
+        m2.reset();
+    }
+    // every position is an L or an S item: the two sorts write the whole SA
+    if (shortL) {
+        const u64 mm = m;
+        const u32 *Li = Litems;
+        scan_exclusive(
+            d, nL,
+            [=] __device__(u64 i) {
+                const u64 q = (u64)Li[i] + 1;
+                return (q >= mm || !getbit(sbits, q)) ? 1u : 0u;
+            },
+            [=] __device__(u64 i, u32 ex, u32 v) {
+                if (v) ordL[m1 + ex] = Li[i];
+            },
+            nullptr);
+        chain_sort(rt, m, RK, ordL, nL, false, rank, tails, k, false, SA, sbits, 1, kpre, m1);
+    } else {
+        chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
+    }
+    labels_S(d, s, m, sbits, RK);
+    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, sbits);
+    mk.reset();
+}
+
 inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA, int depth,
                     const u32 *rank, const u32 *tails, u32 k) {
     Dev &d = rt.d;
@@ -675,86 +803,8 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
         snprintf(what, sizeof(what), "level %d recursion below", depth);
         d.phase(what);
     }
-    labels_L(d, s, m, sbits, RK);
-    // The final L sort's round 1 takes short keys (chain_sort, final) when
-    // the ranks leave room for a run offset beside them: its items are the
-    // predecessors of the sorted LMS suffixes in that order, then the L items
-    // whose successor is L or the sentinel.  cnt is free once the LMS labels
-    // are placed and holds that item order.
-    const bool shortL = rank && m1 && (u64)k < (1ull << 27);
-    u32 *ordL = cnt;
-    u64 *kpre = shortL ? d.a->get<u64>(m1) : nullptr;
-    const int B1 = bit_width(k) + 2;
-    // sorted LMS suffixes are grouped by first symbol already
-    if (m1) {
-        // SA1 holds reduced indices: one record per LMS position in text order
-        // (sequential reads), then one record gather per sorted entry gives
-        // the list and its symbols.  Records are (symbol, position), or with
-        // the short L keys (position, symbol, rank of the predecessor, rank + 1
-        // | the predecessor's L-run offset << 27)
-        ArenaScope m2(*d.a);
-        u64 *vals = d.a->get<u64>(m1);
-        u32 *list = d.a->get<u32>(m1 + 1);
-        u32 *err = d.err;
-        const u32 mm1 = m1;
-        if (shortL) {
-            uint4 *R = d.a->get<uint4>(m1);
-            const u32 *sb = sbits;
-            for_each(d, m1, [=] __device__(u64 p) {
-                const u32 x = lms_text[p];  // an LMS position: x >= 1, x-1 L-type
-                const u32 off = run_offset(sb, x - 1, false);
-                R[p] = make_uint4(x, s[x], rank[x - 1], (rank[x] + 1) | (off << 27));
-            });
-            const int b1 = B1;
-            for_each(d, m1, [=] __device__(u64 j) {
-                const u32 p = SA1[j];
-                uint4 r = make_uint4(0, 0, 0, 0);
-                if (p < mm1) r = R[p];
-                else atomicOr(err, 512u);
-                list[j] = r.x;
-                vals[j] = r.y;
-                kpre[j] = ((u64)r.z << b1) | ((u64)(r.w & ((1u << 27) - 1)) << 2) | 3ull |
-                          ((u64)(r.w >> 27) << RUN_KEY_SHIFT);
-            });
-        } else {
-            u64 *R = d.a->get<u64>(m1);
-            for_each(d, m1, [=] __device__(u64 p) {
-                const u32 x = lms_text[p];
-                R[p] = ((u64)s[x] << 32) | x;
-            });
-            for_each(d, m1, [=] __device__(u64 j) {
-                const u32 p = SA1[j];
-                u64 r = 0;
-                if (p < mm1) r = R[p];
-                else atomicOr(err, 512u);
-                list[j] = (u32)r;
-                vals[j] = r >> 32;
-            });
-        }
-        labels_lms_at_tails(d, s, list, vals, m1, RK, cnt, kpre);
-        if (shortL) for_each(d, m1, [=] __device__(u64 j) { ordL[j] = list[j] - 1; });
-        m2.reset();
-    }
-    // every position is an L or an S item: the two sorts write the whole SA
-    if (shortL) {
-        const u64 mm = m;
-        const u32 *Li = Litems;
-        scan_exclusive(
-            d, nL,
-            [=] __device__(u64 i) {
-                const u64 q = (u64)Li[i] + 1;
-                return (q >= mm || !getbit(sbits, q)) ? 1u : 0u;
-            },
-            [=] __device__(u64 i, u32 ex, u32 v) {
-                if (v) ordL[m1 + ex] = Li[i];
-            },
-            nullptr);
-        chain_sort(rt, m, RK, ordL, nL, false, rank, tails, k, false, SA, sbits, 1, kpre, m1);
-    } else {
-        chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);
-    }
-    labels_S(d, s, m, sbits, RK);
-    chain_sort(rt, m, RK, Sitems, nS, true, rank, tails, k, false, SA, sbits);
+    final_induce32(rt, s, m, sbits, rank, tails, k, lms_text, m1, SA1, Litems, nL, Sitems, nS, RK,
+                   cnt, SA);
     if (d.trace) {
         char what[48];
         snprintf(what, sizeof(what), "level %d final induce", depth);

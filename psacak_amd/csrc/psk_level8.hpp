@@ -117,6 +117,18 @@ __global__ void __launch_bounds__(BLOCK) k_level0_tables(const u32 *stat, u32 *t
     crank[c] = (u8)rk;
 }
 
+// Device bytes of the level-0 chain-sort induce beyond the bit vectors: LMS
+// list and SA1, the bucket-pointer text, ranks, item lists, labels and
+// scratch (20 n), the short round-1 keys of the L sort, and the larger of the
+// LMS records and a chain sort over the larger type class (~43 bytes per
+// item: working copies, group ids and labels, keys, radix and group-sort
+// tables).
+inline u64 chain0_bytes(u64 n, u64 m1, u64 nL) {
+    const u64 A = std::max<u64>(nL, n - nL);
+    return 3 * (n / 8 + 8) + 16 * (m1 + 1) + 20 * (n + 1) + 8 * m1 +
+           std::max<u64>(28 * m1, 43 * A) + (64ull << 20);
+}
+
 // SA of the byte text T[0..n) into SA (device buffers).  With bi, T holds a
 // batch of texts, each with its own sentinel, and SA receives the SA of the
 // concatenation in which every text's suffixes are in their own order.
@@ -183,8 +195,18 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
 
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
     bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
-    u8 *dig = d.a->get<u8>(n);
-    u64 *PW = d.a->get<u64>(n);
+    // Many distinct bytes: the final induce runs as chain sorts
+    // (final_induce32), whose cost does not grow with the alphabet, instead
+    // of two multi-split steps per bucket, when its workspace stays within
+    // the published bound.  The bucket steps' digit cache and byte windows
+    // are then needed only by an induced naming fallback.
+    u32 sigma = 0;
+    for (int c = 0; c < 256; c++) sigma += B.cnt[c] != 0;
+    const bool chain0 = !bi && rt.chain_sigma && sigma >= rt.chain_sigma && n < (1ull << 31) &&
+                        chain0_bytes(n, m1, nL_total) <=
+                            (rt.lms_records ? workspace_bytes8(n) : lean_bytes8(n));
+    u8 *dig = chain0 ? nullptr : d.a->get<u8>(n);
+    u64 *PW = chain0 ? nullptr : d.a->get<u64>(n);
     const Pred pr{T, SB, dbits};
 
     d.phase("level 0 types, buckets, LMS list");
@@ -198,7 +220,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // placement records (filled after the recursion), held from here:
     // allocated after the recursion they would need a chunk of their own that
     // no other buffer uses, call after call
-    LmsRec *R = rt.lms_records ? d.a->get<LmsRec>(m1) : nullptr;
+    LmsRec *R = rt.lms_records && !chain0 ? d.a->get<LmsRec>(m1) : nullptr;
     bool named = false;
     {
         // by sorting the substrings, bytes ranked among those present; in a
@@ -212,8 +234,6 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             ends.EB = EB;
             ends.EBR = EBR;
         }
-        u32 sigma = 0;
-        for (int c = 0; c < 256; c++) sigma += B.cnt[c] != 0;
         named = solve_reduced_direct(rt, ByteRanks{T, crank}, sigma, n, lms_text, m1, SA1, 0, st,
                                      ends);
     }
@@ -222,6 +242,11 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         // index does not fit a key): LMS characters to bucket tails
         // (psacak8.rs:57-72), in text order within a bucket, then induce the
         // LMS substrings
+        ArenaScope ms(*d.a);
+        if (chain0) {
+            dig = d.a->get<u8>(n);
+            PW = d.a->get<u64>(n);
+        }
         multisplit(
             d, m1,
             [=] __device__(u64 i, u32 *key) {
@@ -264,6 +289,44 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         }
         u32 *NP = d.a->get<u32>(n / 2 + 1);
         solve_reduced(rt, T, lms_sorted, lms_text, lmsbits, nwords, NP, m1, SA1, 0, st, EB);
+        ms.reset();
+    }
+
+    if (chain0) {
+        // the text in bucket-pointer form (an L byte -> its bucket's head, an
+        // S byte -> its tail - 1: naming.rs:263-274), the bytes' dense ranks
+        // with the tail of each rank's bucket, and the L and S positions
+        d.phase("level 0 recursion below");
+        ArenaScope mc(*d.a);
+        u32 *s0 = d.a->get<u32>(n), *rank0 = d.a->get<u32>(n);
+        u32 *items = d.a->get<u32>(n);
+        u32 *RK = d.a->get<u32>(n + 1), *cnt = d.a->get<u32>(n + 1);
+        u32 *rtails = d.a->get<u32>(256);
+        const u32 *hd = tables, *tl = tables + 256, *bc = stat;
+        const u8 *cr = crank;
+        for_each(d, 256, [=] __device__(u64 c) {
+            if (bc[c]) rtails[cr[c]] = tl[c];
+        });
+        const u32 *sb = sbits;
+        for_each(d, n, [=] __device__(u64 x) {
+            const u32 c = T[x];
+            s0[x] = getbit(sb, x) ? tl[c] - 1 : hd[c];
+            rank0[x] = cr[c];
+        });
+        const u64 nL = nL_total;
+        scan_exclusive(
+            d, n, [=] __device__(u64 x) { return 1u - getbit(sb, x); },
+            [=] __device__(u64 x, u32 off, u32 v) {
+                if (v) items[off] = (u32)x;
+            },
+            nullptr);
+        positions_of(d, n, [=] __device__(u64 w) { return sb[w]; }, items + nL);
+        final_induce32(rt, s0, n, sbits, rank0, rtails, sigma, lms_text, m1, SA1, items,
+                       (u32)nL, items + nL, (u32)(n - nL), RK, cnt, SA);
+        mc.reset();
+        d.phase("level 0 final induce");
+        mk.reset();
+        return;
     }
 
     d.phase("level 0 records after recursion");

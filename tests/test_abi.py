@@ -105,6 +105,17 @@ def test_doubling_setting_without_gpu():
             assert e.value.code == P.PSK_EINVAL
 
 
+def test_chain_induce_setting_without_gpu():
+    with P.Context() as c:
+        for v in (0, 1, 64, 256):
+            c.set_chain_induce(v)
+        for v in (-1, 257):
+            with pytest.raises(P.PskError) as e:
+                c.set_chain_induce(v)
+            assert e.value.code == P.PSK_EINVAL
+    assert P._lib.psk_ctx_set_chain_induce(None, 1) == P.PSK_EINVAL
+
+
 def test_dict_naming_setting_without_gpu():
     with P.Context() as c:
         c.set_dict_naming(False)

@@ -455,6 +455,58 @@ def test_cli_matches_crate_output_format(tmp_path, capsys):
     assert err[3] == "check: true"
 
 
+@pytest.mark.parametrize("threshold", [1, 0])
+def test_chain_induce_level0(threshold):
+    """Level 0's final induce as chain sorts (threshold 1: every byte text)
+    or as bucket steps (0): the same SA as the oracle on small and large
+    alphabets, runs, periodic and repetitive texts, and the level statistics
+    of either path equal."""
+    note(f"chain induce threshold={threshold}")
+    a, b = b"a", b"ab"
+    while len(b) < 100000:
+        a, b = b, b + a
+    blk = gen.uniform8(1 << 17, alphabet=200, seed=9).tobytes()
+    texts = [gen.markov_dna(1 << 20, seed=11), gen.iid_dna(1 << 19, seed=12), b,
+             gen.uniform8(1 << 20, alphabet=3, seed=13), gen.uniform8(1 << 20, alphabet=40, seed=14),
+             gen.uniform8(1 << 20, alphabet=256, seed=15), ADVERSARIAL["runs"],
+             bytes(range(256)) * 3000, bytes(range(255, -1, -1)) * 3000,
+             blk + blk + blk[:5000], b"\xff" * 70000 + b"\x00" + b"\xfe" * 30000,
+             b"mississippi", b"ab", b"aab", b"ba" * 7]
+    with P.Context() as c:
+        c.set_chain_induce(threshold)
+        for i, t in enumerate(texts):
+            note(f"chain induce threshold={threshold} text {i} n={len(t)}")
+            assert np.array_equal(P.psacak(t, c), oracle.sacak8(t)), i
+    rng = np.random.default_rng(31)
+    with P.Context() as c:
+        c.set_chain_induce(threshold)
+        for it in range(60):
+            n = int(rng.integers(4, 3000))
+            t = rng.integers(0, int(rng.choice([2, 5, 60, 256])), n).astype(np.uint8).tobytes()
+            note(f"chain induce threshold={threshold} random it={it} n={n}")
+            assert np.array_equal(P.psacak(t, c), oracle.sacak8(t)), (it, t)
+
+
+def test_chain_induce_large_alphabet_device_check(ctx):
+    """A 64 MiB text over 256 symbols (the chain path by default) passes the
+    device checker and the oracle's checker, within the workspace estimate."""
+    import torch
+    n = 64 << 20
+    note(f"chain induce 256 symbols n={n}")
+    t = gen.uniform8(n, alphabet=256, seed=16)
+    dev = torch.device("cuda:0")
+    d_t = torch.from_numpy(t).to(dev)
+    d_sa = torch.empty(n, dtype=torch.int32, device=dev)
+    with P.Context() as c:
+        P.psacak8_dev(d_t, d_sa, ctx=c)
+        torch.cuda.synchronize()
+        reserved, peak = c.workspace()
+        assert 0 < reserved <= peak <= P.workspace_bytes8(n), (reserved, peak)
+    assert P.check8_dev(d_t, d_sa, ctx=ctx)
+    # and the oracle's O(n) checker on the host
+    assert oracle.check_sa8(t, d_sa.cpu().numpy().view(np.uint32)) == 0
+
+
 @pytest.mark.parametrize("kind,n", [("iid", 64 << 20), ("markov", 256 << 20)])
 def test_workspace_within_estimate(kind, n):
     """The workspace a call reserves stays within psk_workspace_bytes8, which
