@@ -222,14 +222,25 @@ constexpr u32 CQ_DIRECT = 7;
 // range of cs; tact[t] counts them.  The tile's entry counts go to the
 // count matrix cm (nt tiles per row): row d-1 (d <= CQ_DIRECT) holds the
 // seeds with r >= d, row CQ_DIRECT the entries deeper than CQ_DIRECT.
+// da (optional): the seed count is *da (device-sized launch: the workgroups
+// stride over its tiles), and block 0 leaves the count matrix's size in
+// *msize.
 __global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, const u32 *SA, u64 *PW,
                                                       u8 *dig, QPos qp, u32 c, u64 a, u32 none,
                                                       ChainSeed *cs, u32 *tact, u32 *cm, u32 nt,
-                                                      u32 *maxr) {
+                                                      u32 *maxr, const u32 *da = nullptr,
+                                                      u32 *msize = nullptr) {
     __shared__ u32 red[NWAVES + 1];
     __shared__ u32 sums[CQ_DIRECT + 1];
+    if (da) {
+        a = *da;
+        nt = (u32)((a + CQ_TILE - 1) / CQ_TILE);
+        if (msize && blockIdx.x == 0 && threadIdx.x == 0) *msize = (CQ_DIRECT + 1) * nt;
+    }
+    for (u32 t = blockIdx.x; t < nt; t += gridDim.x) {
+    if (t != blockIdx.x) __syncthreads();  // the previous tile's sums are written
     if (threadIdx.x <= CQ_DIRECT) sums[threadIdx.x] = 0;
-    const u64 base = (u64)blockIdx.x * CQ_TILE;
+    const u64 base = (u64)t * CQ_TILE;
     u32 nact = 0, mx = 0, deep = 0;
     u32 ge[CQ_DIRECT];
     #pragma unroll
@@ -265,12 +276,13 @@ __global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, con
         if (lane_id() == WAVE - 1 && v) atomicAdd(&sums[q], v);
     }
     __syncthreads();
-    if (threadIdx.x <= CQ_DIRECT) cm[(u64)threadIdx.x * nt + blockIdx.x] = sums[threadIdx.x];
-    if (threadIdx.x == 0) tact[blockIdx.x] = nact;
+    if (threadIdx.x <= CQ_DIRECT) cm[(u64)threadIdx.x * nt + t] = sums[threadIdx.x];
+    if (threadIdx.x == 0) tact[t] = nact;
     // same-address atomics serialise in L2: only raise the maximum when this
     // wave beats the value already stored
     const u32 wm = wave_max(mx);
     if (lane_id() == 0 && wm > __atomic_load_n(maxr, __ATOMIC_RELAXED)) atomicMax(maxr, wm);
+    }
 }
 
 // Chain entries of one tile's compacted seeds.  co is the exclusive scan of
@@ -280,14 +292,30 @@ __global__ void __launch_bounds__(BLOCK) k_chain_runs(Pred pr, bool aligned, con
 // is written to its slot (SA, PW, dig); a deeper one is emitted as (depth -
 // CQ_DIRECT - 1, entry, predecessor byte, window) at its index past d0, in
 // seed order, for the split by depth.
+// da (optional): device-sized launch as in k_chain_runs, with d0 = hv[2]
+// and the total of the count matrix in hv[1] (checked against the queue's
+// chain entries); block 0 leaves the number of deep entries in hv[3].
 __global__ void __launch_bounds__(BLOCK) k_chain_emit(Pred pr, const ChainSeed *cs, const u32 *tact,
                                                       const u32 *co, u32 nt, u64 d0, u32 c,
                                                       u32 none, QPos qp, u64 a, u64 limit, u32 *SA,
                                                       u64 *PW, u8 *dig, u32 *ek, u32 *ev, u8 *eb,
-                                                      u64 *ew, u32 *err) {
+                                                      u64 *ew, u32 *err, const u32 *da = nullptr,
+                                                      u32 *hv = nullptr) {
     __shared__ u64 red64[NWAVES + 1];
     __shared__ u32 red[NWAVES + 1];
-    const u32 t = blockIdx.x;
+    if (da) {
+        a = *da;
+        nt = (u32)((a + CQ_TILE - 1) / CQ_TILE);
+        d0 = hv[2];
+        const bool bad = a > limit || (u64)hv[1] != limit - a || d0 > limit - a;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            hv[3] = bad ? 0u : (u32)(limit - a - d0);
+            if (bad) atomicOr(err, 2u);  // chain entries do not fill their bucket part
+        }
+        if (bad) return;
+    }
+    for (u32 t = blockIdx.x; t < nt; t += gridDim.x) {
+    if (t != blockIdx.x) __syncthreads();
     const u64 base = (u64)t * CQ_TILE;
     const u32 na = tact[t];
     u64 off[CQ_DIRECT];
@@ -343,6 +371,7 @@ __global__ void __launch_bounds__(BLOCK) k_chain_emit(Pred pr, const ChainSeed *
         #pragma unroll
         for (u32 q = 0; q < CQ_DIRECT; q++) off[q] += (tot >> (9 * q)) & 511u;
         doff += dtot;
+    }
     }
 }
 
@@ -444,6 +473,79 @@ inline void chain_queue8(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *di
             dig[base + i] = eb[j];
         });
     }
+}
+
+// chain_queue8 without host synchronisation, for texts whose runs of equal
+// bytes are all at most 256 long (every deep entry's depth fits one 8-bit
+// multi-split digit): the seed count a = *d_start, the count matrix's size,
+// the deep entries' start and count stay on the device, and the launches are
+// device-sized from the bucket part's size (limit >= a).  The queue's fill
+// is checked on the device (err bit 2).
+inline void chain_queue8_dn(Dev &d, Pred pr, bool aligned, u32 *SA, u64 *PW, u8 *dig, QPos qp,
+                            u32 c, const u32 *d_start, u64 limit, u32 none) {
+    if (limit == 0) return;
+    u32 *err = d.err;
+    ArenaScope mk(*d.a);
+    const u64 ntb = ceil_div(limit, CQ_TILE);
+    ChainSeed *cs = d.a->get<ChainSeed>(limit);
+    const u64 rows = (u64)(CQ_DIRECT + 1) * ntb;
+    u32 *tact = d.a->get<u32>(ntb), *cm = d.a->get<u32>(rows), *co = d.a->get<u32>(rows);
+    // largest run, entries in all, first deep entry, deep entries, matrix size
+    u32 *hv = d.a->get<u32>(8);
+    d.zero(hv, 8 * sizeof(u32));
+    u32 *msize = hv + 4;
+    // seeds are most of a large bucket part: one workgroup per tile of the
+    // bound (the gathers of a tile are latency-bound)
+    const u32 g = dn_grid(ntb, ntb);
+    {
+        int prec = d.pbegin("chain8_runs", 13.0 * limit);
+        k_chain_runs<<<g, BLOCK, 0, d.s>>>(pr, aligned, SA, PW, dig, qp, c, 0, none, cs, tact, cm,
+                                           0, hv, d_start, msize);
+        PSK_LAUNCH_CHECK();
+        d.pend(prec);
+    }
+    // queue offsets of every (depth, tile) and of the deep entries: the deep
+    // row starts at CQ_DIRECT / (CQ_DIRECT + 1) of the matrix
+    u32 *h2 = hv + 2;
+    scan_exclusive_dn(
+        d, rows, msize, [=] __device__(u64 i) { return cm[i]; },
+        [=] __device__(u64 i, u32 ex, u32) {
+            co[i] = ex;
+            if (i == (u64)*msize / (CQ_DIRECT + 1) * CQ_DIRECT) *h2 = ex;
+        },
+        hv + 1);
+    u32 *ek = d.a->get<u32>(limit), *ev = d.a->get<u32>(limit);
+    u8 *eb = d.a->get<u8>(limit);
+    u64 *ew = d.a->get<u64>(limit);
+    {
+        int pe = d.pbegin("chain8_emit", 29.0 * limit);
+        k_chain_emit<<<g, BLOCK, 0, d.s>>>(pr, cs, tact, co, 0, 0, c, none, qp, 0, limit, SA, PW,
+                                           dig, ek, ev, eb, ew, err, d_start, hv);
+        PSK_LAUNCH_CHECK();
+        d.pend(pe);
+    }
+    // deep entries in breadth-first order: a stable split by depth, after the
+    // seeds and the entries of depth <= CQ_DIRECT
+    const u64 lim = limit;
+    const u32 *dd0 = hv + 2;
+    multisplit_dn(
+        d, limit, hv + 3,
+        [=] __device__(u64 j, u32 *key) {
+            *key = ek[j];
+            return true;
+        },
+        [=] __device__(u64 j, u32, u32 gpos, u32) {
+            const u64 q = (u64)*d_start + *dd0 + gpos;
+            if (q < lim) {
+                const u64 slot = qp(q);
+                SA[slot] = ev[j];
+                PW[slot] = ew[j];
+                dig[q] = eb[j];
+            } else {
+                atomicOr(err, 2u);
+            }
+        });
+    mk.reset();
 }
 
 // The same queue fill without host synchronisation, for texts whose runs of
@@ -592,20 +694,18 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
     const Pred pr{T, SB, DB};
     const bool aligned = (((uintptr_t)T) & 7u) == 0;
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
-    // Small queues (many buckets: large alphabets) are filled without host
-    // synchronisation unless a run of equal bytes exceeds 256 (chain depths
-    // past one multi-split digit); large ones take the exact sizes, whose
-    // readback costs less than the maximum-shape launches would
+    // Queues fill without host synchronisation when no run of equal bytes
+    // exceeds 256 (every chain depth fits one multi-split digit): small ones
+    // (large alphabets) from per-seed launches of the bucket part's shape,
+    // large ones from tiles of seeds with device-side counts; otherwise with
+    // the exact sizes read back
     constexpr u64 SYNC_FREE_MAX = 1ull << 22;
-    bool any_small = false;
-    for (int c = 0; c < 256; c++) {
-        const u64 nl = B.nL[c], ns = B.cnt[c] - B.nL[c];
-        if ((nl && nl < SYNC_FREE_MAX) || (ns && ns < SYNC_FREE_MAX)) any_small = true;
-    }
-    const bool short_runs = any_small && runs_at_most_256(d, DB, n, tb.tmp);
+    const bool short_runs = runs_at_most_256(d, DB, n, tb.tmp);
     auto queue = [&](QPos qp, u32 c, const u32 *d_start, u64 limit, u32 none) {
         if (short_runs && limit < SYNC_FREE_MAX)
             chain_queue8_dev(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);
+        else if (short_runs)
+            chain_queue8_dn(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, none);
         else
             chain_queue8(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);
     };

@@ -415,7 +415,9 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
 // read by the kernels themselves, so the host needs no readback to size the
 // launch: the grid comes from an upper bound the host does know, capped at
 // DN_GRID workgroups, and the workgroups stride over the tiles below the
-// device count.  The cap is a multiple of NXCD, so every tile index a block
+// device count.  (A launch whose bound is close to the device count, with
+// latency-bound tiles, takes one workgroup per tile: cap = the tile count.)
+// The cap is a multiple of NXCD, so every tile index a block
 // strides over is congruent to the block index mod NXCD and xcd_tile keeps
 // each XCD on one contiguous range of tiles.  A device count of 0 costs a
 // launch of empty workgroups (a few microseconds).
@@ -424,8 +426,8 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
 #define PSK_DN_GRID 2048
 #endif
 constexpr u32 DN_GRID = PSK_DN_GRID;  // 2048: 8 workgroups per CU
-inline u32 dn_grid(u64 tiles) {
-    u64 g = tiles < DN_GRID ? tiles : DN_GRID;
+inline u32 dn_grid(u64 tiles, u64 cap = DN_GRID) {
+    u64 g = tiles < cap ? tiles : cap;
     g = (g + NXCD - 1) / NXCD * NXCD;
     return g ? (u32)g : NXCD;
 }
@@ -532,12 +534,12 @@ __device__ __forceinline__ void lds_count_wave(u32 *h, u32 x, bool valid) {
     if (pending) atomicAdd(&h[x], 1u);
 }
 
+// Digit counts of tile t of nt (hist is digit-major: [digit][tile]).  h: 256
+// words of LDS.
 template <class Dig>
-__global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
-    __shared__ u32 h[256];
+__device__ __forceinline__ void ms_hist_tile(Dig dig, u64 n, u32 *hist, u32 nt, u32 t, u32 *h) {
     h[threadIdx.x] = 0;
     __syncthreads();
-    const u32 t = xcd_tile(blockIdx.x, nt);
     u64 base = (u64)t * MS_TILE;
     for (int r = 0; r < MS_ROUNDS; r++) {
         u64 i = base + (u64)r * BLOCK + threadIdx.x;
@@ -549,12 +551,16 @@ __global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
     hist[(u64)threadIdx.x * nt + t] = h[threadIdx.x];
 }
 
+template <class Dig>
+__global__ void k_ms_hist(Dig dig, u64 n, u32 *hist, u32 nt) {
+    __shared__ u32 h[256];
+    ms_hist_tile(dig, n, hist, nt, xcd_tile(blockIdx.x, nt), h);
+}
+
+// Stable scatter of tile t of nt; off: the scanned count matrix.
 template <class Dig, class Put>
-__global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
-    __shared__ u32 run[256];
-    __shared__ u32 dst0[256];
-    __shared__ u32 wc[NWAVES][256];
-    const u32 t = xcd_tile(blockIdx.x, nt);
+__device__ __forceinline__ void ms_scatter_tile(Dig dig, Put put, u64 n, const u32 *off, u32 nt,
+                                                u32 t, u32 *run, u32 *dst0, u32 (*wc)[256]) {
     run[threadIdx.x] = off[(u64)threadIdx.x * nt + t];
     dst0[threadIdx.x] = off[(u64)threadIdx.x * nt];
     #pragma unroll
@@ -583,6 +589,41 @@ __global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
             run[threadIdx.x] += s;
         }
         __syncthreads();
+    }
+}
+
+template <class Dig, class Put>
+__global__ void k_ms_scatter(Dig dig, Put put, u64 n, const u32 *off, u32 nt) {
+    __shared__ u32 run[256];
+    __shared__ u32 dst0[256];
+    __shared__ u32 wc[NWAVES][256];
+    ms_scatter_tile(dig, put, n, off, nt, xcd_tile(blockIdx.x, nt), run, dst0, wc);
+}
+
+// The same with the item count *dn read on the device (device-sized
+// launches); k_ms_hist_dn also leaves the count matrix's size in *msize.
+template <class Dig>
+__global__ void k_ms_hist_dn(Dig dig, const u32 *dn, u32 *hist, u32 *msize) {
+    __shared__ u32 h[256];
+    const u64 n = *dn;
+    const u32 nt = (u32)((n + MS_TILE - 1) / MS_TILE);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *msize = 256 * nt;
+    for (u32 v = blockIdx.x; v < nt; v += gridDim.x) {
+        if (v != blockIdx.x) __syncthreads();
+        ms_hist_tile(dig, n, hist, nt, xcd_tile(v, nt), h);
+    }
+}
+
+template <class Dig, class Put>
+__global__ void k_ms_scatter_dn(Dig dig, Put put, const u32 *dn, const u32 *off) {
+    __shared__ u32 run[256];
+    __shared__ u32 dst0[256];
+    __shared__ u32 wc[NWAVES][256];
+    const u64 n = *dn;
+    const u32 nt = (u32)((n + MS_TILE - 1) / MS_TILE);
+    for (u32 v = blockIdx.x; v < nt; v += gridDim.x) {
+        if (v != blockIdx.x) __syncthreads();
+        ms_scatter_tile(dig, put, n, off, nt, xcd_tile(v, nt), run, dst0, wc);
     }
 }
 
@@ -633,6 +674,26 @@ template <class Dig, class Put>
 inline int multisplit(Dev &d, u64 n, Dig dig, Put put, u32 *d_counts, const char *tag = nullptr,
                       double bytes_count = 0, double bytes_scatter = 0) {
     return multisplit2(d, n, dig, dig, put, d_counts, tag, bytes_count, bytes_scatter);
+}
+
+// Stable multi-split of [0, *dn) (*dn <= ub) with device-side sizes: no
+// host readback (dig and put as in multisplit).
+template <class Dig, class Put>
+inline void multisplit_dn(Dev &d, u64 ub, const u32 *dn, Dig dig, Put put) {
+    if (ub == 0) return;
+    ArenaScope mk(*d.a);
+    const u64 ntb = ceil_div(ub, MS_TILE);
+    u32 *hist = d.a->get<u32>(256 * ntb + 1);
+    u32 *msize = d.a->get<u32>(1);
+    const u32 g = dn_grid(ntb);
+    k_ms_hist_dn<<<g, BLOCK, 0, d.s>>>(dig, dn, hist, msize);
+    PSK_LAUNCH_CHECK();
+    scan_exclusive_dn(
+        d, 256 * ntb, msize, [=] __device__(u64 i) { return hist[i]; },
+        [=] __device__(u64 i, u32 ex, u32) { hist[i] = ex; }, nullptr);
+    k_ms_scatter_dn<<<g, BLOCK, 0, d.s>>>(dig, put, dn, hist);
+    PSK_LAUNCH_CHECK();
+    mk.reset();
 }
 
 // ---------------------------------------------------------------------------
