@@ -455,6 +455,23 @@ def test_cli_matches_crate_output_format(tmp_path, capsys):
     assert err[3] == "check: true"
 
 
+@pytest.mark.parametrize("case", ["markov", "iid", "long_run"])
+def test_large_queues_against_oracle(ctx, case):
+    """Texts large enough for level-0 queues of more than 4M entries: filled
+    without host synchronisation (device-side seed counts) when no run of
+    equal bytes exceeds 256, with exact readbacks when one does (long_run)."""
+    n = 40 << 20  # bucket parts of 5M+ entries
+    note(f"large queues {case} n={n}")
+    if case == "markov":
+        t = gen.markov_dna(n, seed=21)
+    else:
+        t = gen.iid_dna(n, seed=22)
+    if case == "long_run":
+        t = t.copy()
+        t[n // 2: n // 2 + 1000] = ord("G")
+    assert np.array_equal(P.psacak(t, ctx), oracle.sacak8(t))
+
+
 @pytest.mark.parametrize("threshold", [1, 0])
 def test_chain_induce_level0(threshold):
     """Level 0's final induce as chain sorts (threshold 1: every byte text)
