@@ -688,9 +688,13 @@ struct Induce8Tables {
 // u64 per SA slot).  dig: scratch of >= n bytes.
 // Batches (SB != nullptr): K texts concatenated, ends[0..K) their last
 // positions; the L-scan starts from every text's sentinel, in text order.
+// lean: large queues take their exact sizes (readbacks) instead of the
+// sync-free fill, whose seed and deep-entry arrays are sized by the bucket
+// part (~33 bytes per entry): a server's worker contexts keep their
+// workspace to what a job strictly needs.
 inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Induce8Tables tb,
                     u8 *dig, u64 *PW, const u32 *DB, const u32 *SB = nullptr,
-                    const u32 *ends = nullptr, u64 K = 1) {
+                    const u32 *ends = nullptr, u64 K = 1, bool lean = false) {
     const Pred pr{T, SB, DB};
     const bool aligned = (((uintptr_t)T) & 7u) == 0;
     u32 *head = tb.head, *tail = tb.tail, *fill = tb.fill, *counts = tb.counts;
@@ -704,7 +708,7 @@ inline void induce8(Dev &d, const u8 *T, u64 n, u32 *SA, const Buckets8 &B, Indu
     auto queue = [&](QPos qp, u32 c, const u32 *d_start, u64 limit, u32 none) {
         if (short_runs && limit < SYNC_FREE_MAX)
             chain_queue8_dev(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);
-        else if (short_runs)
+        else if (short_runs && !lean)
             chain_queue8_dn(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, none);
         else
             chain_queue8(d, pr, aligned, SA, PW, dig, qp, c, d_start, limit, tb.tmp, none);

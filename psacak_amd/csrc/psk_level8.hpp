@@ -260,7 +260,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
                 PW[dst] = pr.window<false>(p);
             },
             nullptr);
-        induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);
+        induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K, !rt.lms_records);
 
         // sorted LMS substrings = LMS entries in SA order (compact_left
         // psacak8.rs:35): LMS flags of the SA entries packed once, then compacted
@@ -354,7 +354,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
         PSK_LAUNCH_CHECK();
     }
     d.phase("level 0 placement");
-    induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K);
+    induce8(d, T, n, SA, B, tb, dig, PW, dbits, SB, ENDS, K, !rt.lms_records);
     d.phase("level 0 final induce");
     mk.reset();
 }
