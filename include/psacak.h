@@ -169,8 +169,10 @@ int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
 
 /* The last call on ctx: [0] time of its device work (ms, HIP events around
  * the call on its stream), [1] host synchronisations with the stream inside
- * the call (readbacks of small counters that size later launches), [2], [3]
- * reserved (0). */
+ * the call (readbacks of small counters that size later launches), [2] of a
+ * psk_psacak32_raw call, the time of its translation into bucket-pointer form
+ * (translate_sample, psacak32.rs:912-943) within [0], else 0, [3] reserved
+ * (0). */
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
 
 /* ---- kernel profiling ---------------------------------------------------- */

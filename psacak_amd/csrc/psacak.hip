@@ -18,6 +18,7 @@ struct psk_ctx {
     bool ready = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev2 = nullptr;  // end of the translation of a psacak32_raw call
     Arena arena;
     u32 *pinned = nullptr;
     u64 budget = 0;
@@ -42,6 +43,7 @@ void ensure_ready(psk_ctx *c) {
     PSK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PSK_HIP(hipEventCreate(&c->ev0));
     PSK_HIP(hipEventCreate(&c->ev1));
+    PSK_HIP(hipEventCreate(&c->ev2));
     PSK_HIP(hipHostMalloc((void **)&c->pinned, 16384, hipHostMallocDefault));
     PSK_HIP(hipMalloc((void **)&c->rt.d.err, 256));
     PSK_HIP(hipMemset(c->rt.d.err, 0, 256));
@@ -100,16 +102,23 @@ int check32(const void *text, u64 n, const void *sa, u64 sa_len) {
 struct Timer {
     psk_ctx *c;
     u64 syncs0;
+    bool split = false;  // ev2 marks the end of a first phase (translation)
     explicit Timer(psk_ctx *cc) : c(cc), syncs0(cc->rt.d.syncs) {
         c->rt.d.zero(c->rt.d.err, sizeof(u32));
         PSK_HIP(hipEventRecord(c->ev0, c->stream));
     }
+    void mark() {
+        PSK_HIP(hipEventRecord(c->ev2, c->stream));
+        split = true;
+    }
     void stop() {
         PSK_HIP(hipEventRecord(c->ev1, c->stream));
         PSK_HIP(hipEventSynchronize(c->ev1));
-        float ms = 0;
+        float ms = 0, ms0 = 0;
         PSK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (split) PSK_HIP(hipEventElapsedTime(&ms0, c->ev0, c->ev2));
         c->last_ms[0] = ms;
+        c->last_ms[2] = ms0;
         c->last_syncs = c->rt.d.syncs - syncs0;
         if (c->rt.d.trace)
             fprintf(stderr, "[psk] call: %.3f ms, %llu host synchronisations\n", ms,
@@ -152,7 +161,9 @@ void run32_dev(psk_ctx *c, const u32 *d_text, u64 n, u32 *d_sa, bool raw) {
     if (raw) {
         ArenaScope mk(c->arena);
         u32 *tr = c->arena.get<u32>(n);
-        if (translate32_run(c->rt, d_text, n, tr, d_sa)) psacak32_run(c->rt, tr, n, d_sa);
+        const bool go = translate32_run(c->rt, d_text, n, tr, d_sa);
+        t.mark();
+        if (go) psacak32_run(c->rt, tr, n, d_sa);
         mk.reset();
     } else {
         psacak32_run(c->rt, d_text, n, d_sa);
@@ -297,6 +308,7 @@ void psk_ctx_destroy(psk_ctx *ctx) {
         ctx->xfer.release();
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
+        (void)hipEventDestroy(ctx->ev2);
         (void)hipStreamDestroy(ctx->stream);
         (void)hipHostFree(ctx->pinned);
         (void)hipFree(ctx->rt.d.err);
@@ -464,7 +476,7 @@ int psk_last_capture(const psk_ctx *ctx, uint32_t *out, uint64_t cap, uint64_t *
 int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries) {
     if (!ctx || !ms) return PSK_EINVAL;
     int n = std::min(max_entries, 4);
-    const double v[4] = {ctx->last_ms[0], (double)ctx->last_syncs, 0.0, 0.0};
+    const double v[4] = {ctx->last_ms[0], (double)ctx->last_syncs, ctx->last_ms[2], 0.0};
     for (int i = 0; i < n; i++) ms[i] = v[i];
     return n;
 }

@@ -410,8 +410,8 @@ __device__ __forceinline__ u32 xcd_tile(u32 b, u32 nt) {
 }
 
 // ---------------------------------------------------------------------------
-// Device-sized launches.  A count that only the device knows (the items still
-// tied after a sorting round, the tiles of a round's large groups, ...) is
+// Device-sized launches.  A count that only the device knows (the seeds of a
+// level-0 bucket queue, its deep chain entries, a count matrix's size) is
 // read by the kernels themselves, so the host needs no readback to size the
 // launch: the grid comes from an upper bound the host does know, capped at
 // DN_GRID workgroups, and the workgroups stride over the tiles below the
@@ -430,21 +430,6 @@ inline u32 dn_grid(u64 tiles, u64 cap = DN_GRID) {
     u64 g = tiles < cap ? tiles : cap;
     g = (g + NXCD - 1) / NXCD * NXCD;
     return g ? (u32)g : NXCD;
-}
-
-template <typename F>
-__global__ void k_for_each_dn(F f, const u32 *dn) {
-    const u64 n = *dn;
-    const u64 stride = (u64)gridDim.x * BLOCK;
-    for (u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) f(i);
-}
-
-// f(i) for i in [0, *dn), with *dn <= ub
-template <typename F>
-inline void for_each_dn(Dev &d, u64 ub, const u32 *dn, F f) {
-    if (ub == 0) return;
-    k_for_each_dn<<<dn_grid(ceil_div(ub, BLOCK)), BLOCK, 0, d.s>>>(f, dn);
-    PSK_LAUNCH_CHECK();
 }
 
 template <class In>
