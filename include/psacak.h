@@ -92,9 +92,9 @@ int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024);
  * integer levels induce (cost independent of the alphabet), instead of one
  * multi-split step per bucket and scan (the crate's bucket-by-bucket scans,
  * psacak8.rs:127-190, 201-344).  Taken only when its workspace stays within
- * psk_workspace_bytes8.  0 = always bucket steps.  Default 96 (measured
- * crossover on 256 MiB random texts: bucket steps faster up to 64 distinct
- * bytes, chain sorts from 128).  No
+ * psk_workspace_bytes8.  0 = always bucket steps.  The defaults (one for
+ * texts below 64 MiB, one above) are the crossovers the autotuner measured
+ * (python -m psacak_amd.tune, psacak_amd/tuning.json).  No
  * counterpart in the crate; results are identical.  PSK_EINVAL outside
  * [0, 256]. */
 int psk_ctx_set_chain_induce(psk_ctx *ctx, int min_distinct_bytes);

@@ -42,13 +42,16 @@ TUNING = os.path.join(HERE, "tuning.json")
 
 
 def tuned_defines():
-    """Tile constants chosen by the autotuner (psacak_amd/tune.py), if any."""
+    """Tile constants and input-keyed thresholds chosen by the autotuner
+    (psacak_amd/tune.py), if any."""
     if not os.path.exists(TUNING):
         return []
     import json
 
     with open(TUNING) as f:
-        return [f"{k}={int(v)}" for k, v in json.load(f).get("defines", {}).items()]
+        rec = json.load(f)
+    items = list(rec.get("defines", {}).items()) + list(rec.get("runtime", {}).items())
+    return [f"{k}={int(v)}" for k, v in items]
 
 
 def build_hip(verbose: bool = False, out_path: str = LIB, defines=None) -> str:

@@ -333,7 +333,7 @@ int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024) {
 
 int psk_ctx_set_chain_induce(psk_ctx *ctx, int min_distinct_bytes) {
     if (!ctx || min_distinct_bytes < 0 || min_distinct_bytes > 256) return PSK_EINVAL;
-    ctx->rt.chain_sigma = (u32)min_distinct_bytes;
+    ctx->rt.chain_sigma = ctx->rt.chain_sigma_small = (u32)min_distinct_bytes;
     return PSK_OK;
 }
 

@@ -29,6 +29,13 @@
 
 namespace psk {
 
+#ifndef PSK_CHAIN_SIGMA
+#define PSK_CHAIN_SIGMA 96
+#endif
+#ifndef PSK_CHAIN_SIGMA_SMALL
+#define PSK_CHAIN_SIGMA_SMALL 96
+#endif
+
 struct LevelStat {
     u64 m, m1, mL, k, sumlen, csize;
 };
@@ -65,8 +72,13 @@ struct Runtime {
     bool dict_naming = true;
     // a byte text with at least this many distinct bytes takes its level-0
     // final induce as chain sorts (final_induce32) instead of one multi-split
-    // step per bucket and scan; 0 = always bucket steps
-    u32 chain_sigma = 96;
+    // step per bucket and scan; 0 = always bucket steps.  Texts of fewer than
+    // CHAIN_SMALL_N bytes use chain_sigma_small (their bucket steps are
+    // shorter, so launch costs weigh more).  The defaults come from the
+    // autotuner's record (psacak_amd/tuning.json) through the build.
+    u32 chain_sigma = PSK_CHAIN_SIGMA;
+    u32 chain_sigma_small = PSK_CHAIN_SIGMA_SMALL;
+    static constexpr u64 CHAIN_SMALL_N = 1ull << 26;
     // level 0 places its sorted LMS suffixes from 16-byte (position, window)
     // records gathered by reduced index (one random read each, 16 bytes per
     // LMS suffix of extra workspace), or, when false, by gathering the

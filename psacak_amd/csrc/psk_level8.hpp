@@ -202,7 +202,8 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     // are then needed only by an induced naming fallback.
     u32 sigma = 0;
     for (int c = 0; c < 256; c++) sigma += B.cnt[c] != 0;
-    const bool chain0 = !bi && rt.chain_sigma && sigma >= rt.chain_sigma && n < (1ull << 31) &&
+    const u32 csig = n < Runtime::CHAIN_SMALL_N ? rt.chain_sigma_small : rt.chain_sigma;
+    const bool chain0 = !bi && csig && sigma >= csig && n < (1ull << 31) &&
                         chain0_bytes(n, m1, nL_total) <=
                             (rt.lms_records ? workspace_bytes8(n) : lean_bytes8(n));
     u8 *dig = chain0 ? nullptr : d.a->get<u8>(n);

@@ -10,10 +10,20 @@ The tuner searches them one coordinate at a time: for each candidate value it
 builds the library with hipcc (``build.build_hip(defines=...)``) into a
 scratch directory, times a real psacak8 call on a seeded DNA text in a fresh
 process that loads that build (``PSACAK_AMD_LIB``), checks the SA there with
-the device checker, and keeps the fastest.  The result is written to
-``psacak_amd/tuning.json``, which ``build.py`` applies to every later build.
+the device checker, and keeps the fastest.
 
-    python -m psacak_amd.tune [--n BYTES] [--reps K] [--out FILE]
+A second phase tunes choices keyed by the input, with the built library and
+its context switches (no rebuild): for each text size class (below and above
+64 MiB) it times the level-0 final induce both ways -- bucket steps and chain
+sorts (``Context.set_chain_induce``) -- on seeded random texts over a range of
+alphabet sizes, and takes as the class's threshold the smallest alphabet from
+which the chain sorts are faster at every larger one measured
+(``PSK_CHAIN_SIGMA``, ``PSK_CHAIN_SIGMA_SMALL``).
+
+The result is written to ``psacak_amd/tuning.json`` ("defines": tile shapes,
+"runtime": thresholds), which ``build.py`` applies to every later build.
+
+    python -m psacak_amd.tune [--n BYTES] [--reps K] [--out FILE] [--runtime-only]
 """
 from __future__ import annotations
 
@@ -57,6 +67,76 @@ for _ in range(reps):
 ok = P.check8_dev(dt, ds, n, ctx=ctx)
 print(json.dumps({{"ms": min(ms), "all_ms": ms, "ok": bool(ok)}}))
 """
+
+
+# The alphabet sweep of one size class: both final-induce paths per alphabet.
+_CHILD_SIGMA = r"""
+import json, sys
+sys.path.insert(0, {root!r})
+import numpy as np, torch
+import psacak_amd as P
+from psacak_amd import gen
+n, reps, sigmas = {n}, {reps}, {sigmas!r}
+ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+rows = []
+for sig in sigmas:
+    t = gen.uniform8(n, alphabet=sig, seed=7100 + sig)
+    dt = torch.from_numpy(t).to("cuda:0")
+    row = {{"sigma": sig}}
+    for name, thr in (("buckets", 0), ("chain", 1)):
+        with P.Context() as ctx:
+            ctx.set_chain_induce(thr)
+            P.psacak8_dev(dt, ds, n, ctx=ctx)
+            ms = []
+            for _ in range(reps):
+                P.psacak8_dev(dt, ds, n, ctx=ctx)
+                ms.append(float(ctx.last_timing_ms()[0]))
+            row[name] = min(ms)
+            row[name + "_ok"] = bool(P.check8_dev(dt, ds, n, ctx=ctx))
+    rows.append(row)
+    print(json.dumps(row), file=sys.stderr, flush=True)
+print(json.dumps(rows))
+"""
+
+SIGMAS = [4, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256]
+SIZE_CLASSES = {"PSK_CHAIN_SIGMA_SMALL": 1 << 24, "PSK_CHAIN_SIGMA": 1 << 28}
+
+
+def _threshold(rows) -> int:
+    """Smallest alphabet from which the chain sorts beat the bucket steps at
+    every larger alphabet measured (256 + 1 = never, i.e. 0)."""
+    thr = 0
+    for r in sorted(rows, key=lambda r: -r["sigma"]):
+        if r["chain"] < r["buckets"]:
+            thr = r["sigma"]
+        else:
+            break
+    return thr
+
+
+def tune_runtime(reps: int, log=print, lib: str | None = None) -> dict:
+    out = {}
+    sweeps = {}
+    for name, n in SIZE_CLASSES.items():
+        env = dict(os.environ)
+        if lib:
+            env["PSACAK_AMD_LIB"] = lib
+        code = _CHILD_SIGMA.format(root=ROOT, n=n, reps=reps, sigmas=SIGMAS)
+        # the child's per-alphabet lines go straight to stderr (progress)
+        r = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE,
+                           text=True, timeout=1200)
+        if r.returncode != 0:
+            raise RuntimeError(f"sweep failed ({r.returncode})")
+        rows = json.loads(r.stdout.strip().splitlines()[-1])
+        if not all(x["buckets_ok"] and x["chain_ok"] for x in rows):
+            raise RuntimeError(f"an SA of the alphabet sweep failed the device check: {rows}")
+        for x in rows:
+            log(f"[tune] n={n} sigma={x['sigma']}: bucket steps {x['buckets']:.3f} ms, "
+                f"chain sorts {x['chain']:.3f} ms")
+        out[name] = _threshold(rows)
+        sweeps[name] = {"n": n, "rows": rows}
+        log(f"[tune] {name} = {out[name]}")
+    return {"runtime": out, "sweeps": sweeps}
 
 
 def _measure(lib: str, n: int, reps: int) -> dict:
@@ -110,13 +190,33 @@ def tune(n: int, reps: int, out: str, log=print) -> dict:
     return rec
 
 
+def merge_runtime(rt: dict, out: str) -> dict:
+    """Adds the input-keyed thresholds to the record at out (kept otherwise)."""
+    rec = {}
+    if os.path.exists(out):
+        with open(out) as f:
+            rec = json.load(f)
+    elif os.path.exists(TUNING):
+        with open(TUNING) as f:
+            rec = json.load(f)
+    rec["runtime"] = rt["runtime"]
+    rec["runtime_sweeps"] = rt["sweeps"]
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=1 << 28, help="text bytes of the timed call")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=TUNING)
+    ap.add_argument("--runtime-only", action="store_true",
+                    help="only the input-keyed thresholds, with the current build")
     a = ap.parse_args()
-    tune(a.n, a.reps, a.out)
+    if not a.runtime_only:
+        tune(a.n, a.reps, a.out)
+    merge_runtime(tune_runtime(a.reps), a.out)
 
 
 if __name__ == "__main__":
