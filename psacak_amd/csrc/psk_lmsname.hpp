@@ -514,6 +514,7 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
     __syncthreads();
     const u32 mask = (1u << DICT_TBITS) - 1u;
     bool abandon = false;
+    u32 probes = 0;
     for (u64 r = (u64)blockIdx.x * DICT_BLOCK + threadIdx.x; r < m1;
          r += (u64)gridDim.x * DICT_BLOCK) {
         u64 key = lms_key(rk, lms_text, m1, m, (u32)r, 0, D, Db, term, E);
@@ -524,8 +525,14 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
             SLOT[r] = home;
             continue;
         }
+        // an abandoned dictionary's results are discarded: every 16th trip
+        // to the table reads the flag word first (a coherent load bypasses
+        // the XCD's L2, so it is not made per item) and stops once it is set
+        if ((probes++ & 15u) == 0 &&
+            __hip_atomic_load(&ctr[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
         u32 s = home;
-        bool ok = false;
+        bool ok = false, ins = false;
         for (u32 pr = 0; pr < DICT_MAX_PROBES; pr++) {
             // a slot changes once, from empty to its key: a plain load that
             // still sees it empty is corrected by the CAS
@@ -542,13 +549,23 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_dict_insert(PackedRanks rk, cons
                 u64 old = atomicCAS((unsigned long long *)&tkey[s], (unsigned long long)DICT_EMPTY,
                                     (unsigned long long)key);
                 if (old == DICT_EMPTY) {
-                    if (atomicAdd(&ctr[0], 1u) >= DICT_MAX_KEYS) atomicOr(&ctr[1], 4u);
+                    ins = true;
                     ok = true;
                     break;
                 }
                 if (old == key) { ok = true; break; }
             }
             s = (s + 1) & mask;
+        }
+        {
+            // new keys counted once per wave (the lanes that probed): a
+            // text with millions of distinct keys would otherwise send one
+            // atomic per key to the same counter
+            const u64 b = ballot(ins);
+            if (ins && lane_id() == __ffsll((long long)b) - 1) {
+                const u32 c = (u32)__popcll(b);
+                if (atomicAdd(&ctr[0], c) + c > DICT_MAX_KEYS) atomicOr(&ctr[1], 4u);
+            }
         }
         if (abandon) break;
         if (ok) {
