@@ -486,8 +486,14 @@ inline bool name_lms_direct(Dev &d, RF rk, u64 sigma, u64 m, const u32 *lms_text
 // with their group's dense id and first slot.
 // ---------------------------------------------------------------------------
 constexpr u64 DICT_EMPTY = ~0ull;  // keys never have bit 63 set
-constexpr int DICT_TBITS = 22;     // insert table of 2^22 slots
-constexpr u32 DICT_MAX_KEYS = 1u << 21;
+#ifndef PSK_DICT_TBITS
+#define PSK_DICT_TBITS 19
+#endif
+// insert table of 2^DICT_TBITS slots (4 MB), at most a quarter of them
+// filled: a text with more distinct keys names faster by sorting (its keys
+// do not fit the workgroups' LDS caches, so every substring probes the table)
+constexpr int DICT_TBITS = PSK_DICT_TBITS;
+constexpr u32 DICT_MAX_KEYS = 1u << (DICT_TBITS - 2);
 constexpr u32 DICT_MAX_PROBES = 128;
 constexpr int DICT_LDS = 8192;     // per-workgroup cache of inserted keys (two places each)
 constexpr int DICT_BLOCK = 1024;   // insert and ranking workgroups (64 KB of LDS each)
