@@ -84,8 +84,8 @@ int psk_ctx_set_dict_naming(psk_ctx *ctx, int on);
  * recurses, psacak32.rs:24-56; results are identical).  A level whose
  * suffixes keep tying (long repeats: ties that stop shrinking after 2 rounds,
  * or more than an eighth tied after 4) recurses after all, and so do the
- * levels below it.  0 = always recurse.  Default 128.  PSK_EINVAL outside
- * [0, 1024]. */
+ * levels below it.  0 = always recurse.  The default is the autotuner's
+ * choice (psacak_amd/tuning.json).  PSK_EINVAL outside [0, 1024]. */
 int psk_ctx_set_doubling(psk_ctx *ctx, int distinct_per_1024);
 /* Final induce of a byte text (level 0) with at least min_distinct_bytes
  * distinct bytes: as two chain sorts over its L and S positions, the way the
@@ -181,6 +181,9 @@ int psk_last_timing(const psk_ctx *ctx, double *ms, int max_entries);
  * of the tagged kernels launched by this context (and clears old totals);
  * on == 0 stops. */
 int psk_profile(psk_ctx *ctx, int on);
+/* Records only the tags in names (comma-separated; NULL or "" = every tag),
+ * so a timed run can bracket its dominant kernel alone with events. */
+int psk_profile_only(psk_ctx *ctx, const char *names);
 /* Totals per tagged kernel since psk_profile(ctx, 1): names are written to
  * names ('\n'-separated, NUL-terminated, names_len bytes), and launches[i],
  * ms[i], bytes[i] for i < max.  Returns the number of kernels. */

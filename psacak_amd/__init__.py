@@ -89,6 +89,7 @@ def _load():
         "psk_batch8_dev": ([vp, vp, vp, u64, vp, vp], i32),
         "psk_check8_dev": ([vp, vp, u64, vp, ctypes.POINTER(i32), vp], i32),
         "psk_profile": ([vp, i32], i32),
+        "psk_profile_only": ([vp, ctypes.c_char_p], i32),
         "psk_server_create": ([ctypes.POINTER(vp), u64, i32], i32),
         "psk_server_footprint": ([u64, i32], u64),
         "psk_server_submit": ([vp, vp, u64, ctypes.POINTER(u64)], i32),
@@ -389,9 +390,11 @@ def check32_dev(d_text, d_sa, n: int | None = None, ctx: Context | None = None,
     return bool(ok.value)
 
 
-def kernel_profile(ctx: Context, on: bool) -> None:
+def kernel_profile(ctx: Context, on: bool, only=None) -> None:
     """Start (clearing old totals) or stop HIP-event profiling of the tagged
-    kernels launched by ``ctx``."""
+    kernels launched by ``ctx``; ``only`` (names) limits it to those tags."""
+    names = ",".join(only) if only else ""
+    ctx._check(_lib.psk_profile_only(ctx.handle, names.encode()))
     ctx._check(_lib.psk_profile(ctx.handle, 1 if on else 0))
 
 

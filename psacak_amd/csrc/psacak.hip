@@ -491,6 +491,23 @@ int psk_profile(psk_ctx *ctx, int on) {
     });
 }
 
+int psk_profile_only(psk_ctx *ctx, const char *names) {
+    if (!ctx) return PSK_EINVAL;
+    return guarded(ctx, [&] {
+        ctx->prof.only.clear();
+        std::string cur;
+        for (const char *p = names ? names : ""; ; p++) {
+            if (*p == ',' || *p == '\0') {
+                if (!cur.empty()) ctx->prof.only.push_back(cur);
+                cur.clear();
+                if (*p == '\0') break;
+            } else {
+                cur += *p;
+            }
+        }
+    });
+}
+
 int psk_profile_stats(psk_ctx *ctx, char *names, uint64_t names_len, uint64_t *launches,
                       double *ms, double *bytes, int max) {
     if (!ctx || !names || names_len == 0 || max < 0) return PSK_EINVAL;

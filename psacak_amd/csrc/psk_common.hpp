@@ -261,9 +261,14 @@ class Prof {
 
     ~Prof() { release(); }
 
+    // tags to record (empty: all); a launch of another tag takes no events
+    std::vector<std::string> only;
+
     // returns the record index, or -1 when profiling is off
     int begin(hipStream_t s, const char *name, double static_bytes) {
         if (!on) return -1;
+        if (!only.empty() && std::find(only.begin(), only.end(), std::string(name)) == only.end())
+            return -1;
         if (!dyn_) {
             if (hipMalloc((void **)&dyn_, kMaxRecs * sizeof(u64)) != hipSuccess) { on = false; return -1; }
             if (hipMemset(dyn_, 0, kMaxRecs * sizeof(u64)) != hipSuccess) { on = false; return -1; }

@@ -32,6 +32,9 @@ namespace psk {
 #ifndef PSK_CHAIN_SIGMA
 #define PSK_CHAIN_SIGMA 96
 #endif
+#ifndef PSK_DOUBLING
+#define PSK_DOUBLING 16
+#endif
 #ifndef PSK_CHAIN_SIGMA_SMALL
 #define PSK_CHAIN_SIGMA_SMALL 96
 #endif
@@ -58,8 +61,8 @@ struct Runtime {
     int direct_rounds = 32;
     // a recursion level whose text has at least this fraction of distinct
     // symbols (in 1/1024) is sorted by prefix doubling instead of recursing;
-    // 0 = always recurse
-    u32 doubling_distinct = 128;
+    // 0 = always recurse (default from the autotuner's record)
+    u32 doubling_distinct = PSK_DOUBLING;
     bool doubling_gave_up = false;  // in this call (reset by each entry point)
     int depth = 0;                  // recursion level being solved (traces)
     // capture_depth >= 0: the translated reduced text of that level (the

@@ -20,6 +20,12 @@ alphabet sizes, and takes as the class's threshold the smallest alphabet from
 which the chain sorts are faster at every larger one measured
 (``PSK_CHAIN_SIGMA``, ``PSK_CHAIN_SIGMA_SMALL``).
 
+The same phase times the prefix-doubling threshold (the share of distinct
+symbols from which a recursion level is sorted by prefix doubling instead of
+recursing, ``Context.set_doubling``) on Markov DNA and random texts over 12,
+24 and 64 symbols, and keeps the candidate with the least total time
+(``PSK_DOUBLING``).
+
 The result is written to ``psacak_amd/tuning.json`` ("defines": tile shapes,
 "runtime": thresholds), which ``build.py`` applies to every later build.
 
@@ -99,6 +105,62 @@ print(json.dumps(rows))
 """
 
 SIGMAS = [4, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160, 192, 256]
+
+# The doubling sweep: every candidate threshold on every text, alternating.
+_CHILD_DOUBLING = r"""
+import json, sys
+sys.path.insert(0, {root!r})
+import numpy as np, torch
+import psacak_amd as P
+from psacak_amd import gen
+n, reps, cands = {n}, {reps}, {cands!r}
+texts = {{"markov": gen.markov_dna(n, seed=20260101, repeat_frac=0.10)}}
+for s in (12, 24, 64):
+    texts["k%d" % s] = gen.uniform8(n, alphabet=s, seed=7100 + s)
+ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+rows = []
+for name, t in texts.items():
+    dt = torch.from_numpy(t).to("cuda:0")
+    row = {{"text": name}}
+    for thr in cands + cands:
+        with P.Context() as ctx:
+            ctx.set_doubling(thr)
+            P.psacak8_dev(dt, ds, n, ctx=ctx)
+            ms = []
+            for _ in range(reps):
+                P.psacak8_dev(dt, ds, n, ctx=ctx)
+                ms.append(float(ctx.last_timing_ms()[0]))
+            ok = bool(P.check8_dev(dt, ds, n, ctx=ctx))
+        key = str(thr)
+        row[key] = min(row.get(key, 1e30), min(ms))
+        row[key + "_ok"] = row.get(key + "_ok", True) and ok
+    rows.append(row)
+    print(json.dumps(row), file=sys.stderr, flush=True)
+print(json.dumps(rows))
+"""
+
+DOUBLING = [16, 32, 64, 128, 256]
+
+
+def tune_doubling(reps: int, n: int = 1 << 28, log=print, lib: str | None = None) -> tuple:
+    env = dict(os.environ)
+    if lib:
+        env["PSACAK_AMD_LIB"] = lib
+    code = _CHILD_DOUBLING.format(root=ROOT, n=n, reps=reps, cands=DOUBLING)
+    r = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE,
+                       text=True, timeout=1800)
+    if r.returncode != 0:
+        raise RuntimeError(f"doubling sweep failed ({r.returncode})")
+    rows = json.loads(r.stdout.strip().splitlines()[-1])
+    if not all(x[str(c) + "_ok"] for x in rows for c in DOUBLING):
+        raise RuntimeError(f"an SA of the doubling sweep failed the device check: {rows}")
+    total = {c: sum(x[str(c)] for x in rows) for c in DOUBLING}
+    for c in DOUBLING:
+        log(f"[tune] doubling {c}/1024: " + ", ".join(f"{x['text']} {x[str(c)]:.2f}" for x in rows)
+            + f" ms; total {total[c]:.2f} ms")
+    best = min(DOUBLING, key=lambda c: total[c])
+    log(f"[tune] PSK_DOUBLING = {best}")
+    return best, {"n": n, "rows": rows}
 SIZE_CLASSES = {"PSK_CHAIN_SIGMA_SMALL": 1 << 24, "PSK_CHAIN_SIGMA": 1 << 28}
 
 
@@ -136,6 +198,7 @@ def tune_runtime(reps: int, log=print, lib: str | None = None) -> dict:
         out[name] = _threshold(rows)
         sweeps[name] = {"n": n, "rows": rows}
         log(f"[tune] {name} = {out[name]}")
+    out["PSK_DOUBLING"], sweeps["PSK_DOUBLING"] = tune_doubling(reps, log=log, lib=lib)
     return {"runtime": out, "sweeps": sweeps}
 
 
