@@ -299,7 +299,7 @@ def bench_batch(args):
     for _ in range(args.warmup):
         P.batch8_dev(d_text, off[:101], d_sa, ctx=ctx)
     torch.cuda.synchronize()
-    _, dom_name = profile_one_call(P, ctx, lambda: P.batch8_dev(d_text, off, d_sa, ctx=ctx))
+    kall, dom_name = profile_one_call(P, ctx, lambda: P.batch8_dev(d_text, off, d_sa, ctx=ctx))
     log(f"timing batched x{args.steps}")
     P.kernel_profile(ctx, True, only=[dom_name] if dom_name else None)
     t0 = time.perf_counter()
