@@ -764,17 +764,25 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
                                                       const u32 *__restrict__ vi,
                                                       u64 *__restrict__ ko, u32 *__restrict__ vo,
                                                       u64 n, int shift, const u32 *off, u32 nt) {
+    // The (round, wave, digit) counts are dead once every item holds its
+    // staging slot, so the staged values reuse their memory: 34 KB of LDS per
+    // workgroup, four workgroups per CU (three with separate arrays; a 32 KB
+    // layout with five measured slower)
     __shared__ u64 sk[RS_TILE];
-    __shared__ u32 sv[RS_TILE];
+    __shared__ union {
+        unsigned short wcnt[RS_ROUNDS][NWAVES][256];
+        u32 sv[RS_TILE];
+    } sw;
     __shared__ u32 tstart[256], gbase[256];
-    __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
     __shared__ u32 red[NWAVES + 1];
+    auto &wcnt = sw.wcnt;
     const u32 t = xcd_tile(blockIdx.x, nt);
     const u32 dg = threadIdx.x;
     #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++)
         #pragma unroll
         for (int w = 0; w < NWAVES; w++) wcnt[r][w][dg] = 0;
+    u32 my_start, my_base;  // this thread's digit: start in the tile, global start
     {
         u32 g, c;
         if (SMALL) {
@@ -794,9 +802,8 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
             c = off[o + 1] - g;
         }
         u32 tot;
-        u32 ex = block_exclusive_sum(c, red, &tot);  // includes a barrier
-        tstart[dg] = ex;
-        gbase[dg] = g;
+        my_start = block_exclusive_sum(c, red, &tot);  // includes a barrier
+        my_base = g;
     }
     const u64 base = (u64)t * RS_TILE;
     const int w = wave_id();
@@ -822,7 +829,7 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
     }
     __syncthreads();
     {
-        u32 run = tstart[dg];
+        u32 run = my_start;
         #pragma unroll
         for (int r = 0; r < RS_ROUNDS; r++)
             #pragma unroll
@@ -834,11 +841,16 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const u64 *__restrict__ ki
     }
     __syncthreads();
     #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) rk[r] = (unsigned short)(wcnt[r][w][dk[r]] + rk[r]);
+    __syncthreads();  // every count read: sv takes their memory
+    u32 *sv = sw.sv;
+    tstart[dg] = my_start;
+    gbase[dg] = my_base;
+    #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         if (vv[r]) {
-            u32 lp = (u32)wcnt[r][w][dk[r]] + rk[r];
-            sk[lp] = key[r];
-            sv[lp] = val[r];
+            sk[rk[r]] = key[r];
+            sv[rk[r]] = val[r];
         }
     }
     __syncthreads();
@@ -1127,12 +1139,16 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
                                                       u64 *__restrict__ ko, u32 *__restrict__ vo,
                                                       int shift, const u32 *off, u32 nt,
                                                       Tiles tiles) {
+    // LDS laid out as in k_rs_scatter (sv over the dead counts)
     __shared__ u64 sk[RS_TILE];
-    __shared__ u32 sv[RS_TILE];
+    __shared__ union {
+        unsigned short wcnt[RS_ROUNDS][NWAVES][256];
+        u32 sv[RS_TILE];
+    } sw;
     __shared__ u32 tstart[256];
     __shared__ u64 gbase[256];
-    __shared__ unsigned short wcnt[RS_ROUNDS][NWAVES][256];
     __shared__ u32 red[NWAVES + 1];
+    auto &wcnt = sw.wcnt;
     const u32 t = xcd_tile(blockIdx.x, nt);
     u64 base, mbase, ob;
     u32 cnt, ms;
@@ -1142,6 +1158,8 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
     for (int r = 0; r < RS_ROUNDS; r++)
         #pragma unroll
         for (int w = 0; w < NWAVES; w++) wcnt[r][w][dg] = 0;
+    u32 my_start;
+    u64 my_base;
     {
         // a group's entries are tb*256 + d*nt_g + j: the entry after (d, j)
         // is (d, j+1), or (d+1, 0), or the next group's first entry (the
@@ -1150,9 +1168,8 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
         const u32 g = off[o];
         const u32 cn = off[o + 1] - g;
         u32 tot;
-        const u32 ex = block_exclusive_sum(cn, red, &tot);  // includes a barrier
-        tstart[dg] = ex;
-        gbase[dg] = (u64)g + ob;
+        my_start = block_exclusive_sum(cn, red, &tot);  // includes a barrier
+        my_base = (u64)g + ob;
     }
     const int w = wave_id();
     u64 key[RS_ROUNDS];
@@ -1177,7 +1194,7 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
     }
     __syncthreads();
     {
-        u32 run = tstart[dg];
+        u32 run = my_start;
         #pragma unroll
         for (int r = 0; r < RS_ROUNDS; r++)
             #pragma unroll
@@ -1189,11 +1206,16 @@ __global__ void __launch_bounds__(BLOCK) k_gs_scatter(const u64 *__restrict__ ki
     }
     __syncthreads();
     #pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) rk[r] = (unsigned short)(wcnt[r][w][dk[r]] + rk[r]);
+    __syncthreads();  // every count read: sv takes their memory
+    u32 *sv = sw.sv;
+    tstart[dg] = my_start;
+    gbase[dg] = my_base;
+    #pragma unroll
     for (int r = 0; r < RS_ROUNDS; r++) {
         if (vv[r]) {
-            u32 lp = (u32)wcnt[r][w][dk[r]] + rk[r];
-            sk[lp] = key[r];
-            sv[lp] = val[r];
+            sk[rk[r]] = key[r];
+            sv[rk[r]] = val[r];
         }
     }
     __syncthreads();
