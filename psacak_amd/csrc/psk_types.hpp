@@ -185,6 +185,54 @@ __global__ void __launch_bounds__(BLOCK) k_type_masks(const C *t, u64 n, u64 nwo
     }
 }
 
+// D and V words with 16-byte loads (text 16-byte aligned): per step lane l
+// holds P = 16 / sizeof(C) consecutive symbols of the wave's 64 P, the
+// symbol after its last one comes from lane l + 1 (lane 63 loads it), and
+// the 32 / P lanes of a word OR their bit groups together.
+template <class C>
+__global__ void __launch_bounds__(BLOCK) k_type_masks_v(const C *t, u64 n, u64 nwords,
+                                                       const u32 *endbits, u32 *Dw, u32 *Vw) {
+    constexpr int P = 16 / (int)sizeof(C);
+    constexpr int LPW = 32 / P;
+    static_assert(P * (int)sizeof(C) == 16 && LPW * P == 32, "symbol width");
+    const int l = lane_id();
+    const u64 nsteps = (n + 64 * P - 1) / (64 * P);
+    const u64 nw = (u64)gridDim.x * NWAVES;
+    for (u64 st = (u64)blockIdx.x * NWAVES + wave_id(); st < nsteps; st += nw) {
+        const u64 p0 = (st * WAVE + l) * P;
+        C c[P];
+        if (p0 + P <= n) {
+            const uint4 v = *(const uint4 *)(t + p0);
+            __builtin_memcpy(c, &v, 16);
+        } else {
+            #pragma unroll
+            for (int j = 0; j < P; j++) c[j] = p0 + j < n ? t[p0 + j] : (C)0;
+        }
+        C nx = (C)__shfl_down((u32)c[0], 1, WAVE);
+        if (l == WAVE - 1) nx = p0 + P < n ? t[p0 + P] : (C)0;
+        u32 dm = 0, vm = 0;
+        #pragma unroll
+        for (int j = 0; j < P; j++) {
+            const C a = c[j], b = j + 1 < P ? c[j + 1] : nx;
+            const bool last = p0 + j + 1 >= n;  // p == n-1 is L-type, p >= n padding (L)
+            dm |= (u32)(last || a != b) << j;
+            vm |= (u32)(!last && a < b) << j;
+        }
+        u32 D = dm << (P * (l % LPW)), V = vm << (P * (l % LPW));
+        #pragma unroll
+        for (int o = 1; o < LPW; o <<= 1) {
+            D |= __shfl_xor(D, o, WAVE);
+            V |= __shfl_xor(V, o, WAVE);
+        }
+        const u64 w = (st * WAVE + l) / LPW;
+        if (l % LPW == 0 && w < nwords) {
+            apply_ends(endbits, w * 32, n, D, V);
+            Dw[w] = D;
+            Vw[w] = V;
+        }
+    }
+}
+
 template <class M>
 __global__ void k_type_agg(M masks, u8 *blk_has, u8 *blk_first) {
     __shared__ u32 lh[NWAVES], lf[NWAVES];
@@ -306,11 +354,19 @@ inline void compute_types(Dev &d, const C *t, u64 n, u32 *sbits, const u32 *endb
         k_type_final<<<nb, BLOCK, 0, d.s>>>(masks, bc, sbits, nwords, dbits);
         PSK_LAUNCH_CHECK();
     };
-    if constexpr (sizeof(C) == 1) {
+    const bool vec = (sizeof(C) == 1 || sizeof(C) == 4) && (((uintptr_t)t) & 15u) == 0;
+    if (sizeof(C) == 1 && !vec) {
         run(MasksFromText<C>{t, n, endbits});
     } else {
         u32 *Dw = d.a->get<u32>(nwords), *Vw = d.a->get<u32>(nwords);
-        k_type_masks<C><<<ceil_div(nwords, (u64)BLOCK), BLOCK, 0, d.s>>>(t, n, nwords, endbits, Dw, Vw);
+        if (vec) {
+            const u64 steps = ceil_div(n, (u64)WAVE * (16 / sizeof(C)));
+            const u64 blocks = std::min<u64>(ceil_div(steps, (u64)NWAVES), 65536);
+            k_type_masks_v<C><<<(u32)blocks, BLOCK, 0, d.s>>>(t, n, nwords, endbits, Dw, Vw);
+        } else {
+            k_type_masks<C><<<ceil_div(nwords, (u64)BLOCK), BLOCK, 0, d.s>>>(t, n, nwords, endbits,
+                                                                           Dw, Vw);
+        }
         PSK_LAUNCH_CHECK();
         run(MasksFromWords{Dw, Vw, nwords});
     }
