@@ -121,24 +121,7 @@ inline u64 workspace_bytes8(u64 n) { return lean_bytes8(n) + 8 * n; }
 // positions p < n whose bit in wordfn(w) is set, ascending, into out.
 template <class WF>
 inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
-    u64 nwords = (n + 31) / 32;
-    scan_exclusive(
-        d, nwords,
-        [=] __device__(u64 w) {
-            u32 x = wordfn(w);
-            if (w * 32 + 32 > n) x &= (n - w * 32 >= 32) ? 0xFFFFFFFFu : ((1u << (n - w * 32)) - 1u);
-            return (u32)__popc(x);
-        },
-        [=] __device__(u64 w, u32 off, u32) {
-            u32 x = wordfn(w);
-            if (w * 32 + 32 > n) x &= (n - w * 32 >= 32) ? 0xFFFFFFFFu : ((1u << (n - w * 32)) - 1u);
-            while (x) {
-                int b = __ffs(x) - 1;
-                x &= x - 1;
-                out[off++] = (u32)(w * 32 + b);
-            }
-        },
-        nullptr);
+    bit_list(d, n, wordfn, out, nullptr);
 }
 
 // Sort items[0..A) by chain keys (see above); RK holds the labels of every
@@ -748,18 +731,13 @@ inline void level32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, u32 *SA,
     // L item list (ascending positions); then the LMS and L counts in one read
     u32 *Litems = d.a->get<u32>(m);
     u32 *dcnt = ctr + 1;
-    scan_exclusive(
-        d, m, [=] __device__(u64 x) { return 1u - getbit(sbits, x); },
-        [=] __device__(u64 x, u32 off, u32 v) {
-            if (v) Litems[off] = (u32)x;
-        },
-        dcnt);
+    bit_list(d, m, [=] __device__(u64 w) { return ~sbits[w]; }, Litems, dcnt);
     u32 cts[2];
     d.read(cts, ctr, sizeof(cts));
     const u32 m1 = cts[0], nL = cts[1];
     const u32 nS = (u32)m - nL;
     u32 *lms_text = d.a->get<u32>(m1 + 1);
-    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
+    bits_to_positions(d, lmsbits, nwords, lms_text);
     u32 *Sitems = Litems + nL;
     positions_of(d, m, [=] __device__(u64 w) { return sbits[w]; }, Sitems);
 

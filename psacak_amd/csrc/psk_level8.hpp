@@ -194,7 +194,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
     PSK_LAUNCH_CHECK();
 
     u32 *lms_text = d.a->get<u32>((u64)m1 + 1);
-    bits_to_positions(d, lmsbits, nwords, lms_text, nullptr);
+    bits_to_positions(d, lmsbits, nwords, lms_text);
     // Many distinct bytes: the final induce runs as chain sorts
     // (final_induce32), whose cost does not grow with the alphabet, instead
     // of two multi-split steps per bucket, when its workspace stays within
@@ -315,12 +315,7 @@ inline void psacak8_run(Runtime &rt, const u8 *T, u64 n, u32 *SA, const BatchInf
             rank0[x] = cr[c];
         });
         const u64 nL = nL_total;
-        scan_exclusive(
-            d, n, [=] __device__(u64 x) { return 1u - getbit(sb, x); },
-            [=] __device__(u64 x, u32 off, u32 v) {
-                if (v) items[off] = (u32)x;
-            },
-            nullptr);
+        bit_list(d, n, [=] __device__(u64 w) { return ~sb[w]; }, items, nullptr);
         positions_of(d, n, [=] __device__(u64 w) { return sb[w]; }, items + nL);
         final_induce32(rt, s0, n, sbits, rank0, rtails, sigma, lms_text, m1, SA1, items,
                        (u32)nL, items + nL, (u32)(n - nL), RK, cnt, SA);

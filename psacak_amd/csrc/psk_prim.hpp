@@ -336,6 +336,106 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
     mk.reset();
 }
 
+// Positions p < nbits whose bit is set in wordfn(p / 32), ascending, into
+// out; *d_total (device, may be null) receives their number.  Per tile of
+// SCAN_TILE words the words' counts are scanned wave by wave and every wave
+// writes its positions as one contiguous run, 64 consecutive outputs per
+// store: lane j finds the word holding output j by a binary search over the
+// wave's offsets and the bit by a binary select (a thread writing its own
+// word's positions would store at a stride of the word's count).
+template <bool FUSED, class WF>
+__global__ void __launch_bounds__(BLOCK) k_bits_emit(WF wordfn, u64 nbits, u64 nwords,
+                                                     const u32 *ta, u32 *out, u32 *total) {
+    __shared__ u32 lds[NWAVES + 1];
+    const u64 base = (u64)blockIdx.x * SCAN_TILE;
+    u32 run;
+    if (FUSED) {
+        u32 acc = 0;
+        for (u32 i = threadIdx.x; i < blockIdx.x; i += BLOCK) acc += ta[i];
+        block_incl<OpSum>(acc, lds, &run);  // includes barriers
+    } else {
+        run = ta[blockIdx.x];
+    }
+    const int l = lane_id(), w = wave_id();
+    for (int r = 0; r < SCAN_ROUNDS; r++) {
+        const u64 wb = base + (u64)r * BLOCK + (u64)w * WAVE;  // the wave's first word
+        const u64 wi = wb + l;
+        u32 x = wi < nwords ? wordfn(wi) : 0u;
+        const u64 p0 = wi * 32;
+        if (p0 >= nbits) x = 0;
+        else if (nbits - p0 < 32) x &= (1u << (nbits - p0)) - 1u;
+        const u32 c = (u32)__popc(x);
+        const u32 inc = wave_inclusive_sum(c);
+        const u32 start = inc - c;
+        const u32 wtot = __shfl(inc, WAVE - 1, WAVE);
+        if (l == 0) lds[w] = wtot;
+        __syncthreads();
+        u32 woff = run, btot = 0;
+        #pragma unroll
+        for (int w2 = 0; w2 < NWAVES; w2++) {
+            const u32 t = lds[w2];
+            woff += w2 < w ? t : 0u;
+            btot += t;
+        }
+        __syncthreads();
+        run += btot;
+        for (u32 j0 = 0; j0 < wtot; j0 += WAVE) {  // wave-uniform trip count
+            const u32 j = j0 + (u32)l;
+            int src = 0;
+            #pragma unroll
+            for (int step = WAVE / 2; step >= 1; step >>= 1) {
+                const u32 st = __shfl(start, src + step, WAVE);
+                if (st <= j) src += step;
+            }
+            u32 xs = __shfl(x, src, WAVE);
+            u32 k = j - __shfl(start, src, WAVE);
+            u32 pos = 0;
+            #pragma unroll
+            for (int width = 16; width >= 1; width >>= 1) {
+                const u32 cnt = (u32)__popc(xs & ((1u << width) - 1u));
+                if (k >= cnt) {
+                    k -= cnt;
+                    xs >>= width;
+                    pos += (u32)width;
+                }
+            }
+            if (j < wtot) out[woff + j] = (u32)((wb + (u64)src) * 32 + pos);
+        }
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = run;
+}
+
+template <class WF>
+inline void bit_list(Dev &d, u64 nbits, WF wordfn, u32 *out, u32 *d_total) {
+    const u64 nwords = (nbits + 31) / 32;
+    if (nwords == 0) {
+        if (d_total) d.zero(d_total, sizeof(u32));
+        return;
+    }
+    ArenaScope mk(*d.a);
+    const u32 nt = ceil_div(nwords, SCAN_TILE);
+    u32 *ta = d.a->get<u32>(nt);
+    if (nt > 1) {
+        auto cnt = [=] __device__(u64 w) -> u32 {
+            u32 x = wordfn(w);
+            const u64 p0 = w * 32;
+            if (nbits - p0 < 32) x &= (1u << (nbits - p0)) - 1u;
+            return (u32)__popc(x);
+        };
+        k_scan_reduce<OpSum><<<nt, BLOCK, 0, d.s>>>(cnt, nwords, ta);
+        PSK_LAUNCH_CHECK();
+    }
+    if (nt <= SCAN_FUSE_MAX) {
+        k_bits_emit<true><<<nt, BLOCK, 0, d.s>>>(wordfn, nbits, nwords, ta, out, d_total);
+    } else {
+        k_scan_tiles<OpSum><<<1, BLOCK, 0, d.s>>>(ta, nt, nullptr);
+        PSK_LAUNCH_CHECK();
+        k_bits_emit<false><<<nt, BLOCK, 0, d.s>>>(wordfn, nbits, nwords, ta, out, d_total);
+    }
+    PSK_LAUNCH_CHECK();
+    mk.reset();
+}
+
 // Exclusive sum scan for a compaction: out is called only for the items of
 // tiles with a nonzero sum, so it must act only on items with v != 0 (a
 // sparse predicate then costs one read of the items, not two).

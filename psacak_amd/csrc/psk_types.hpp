@@ -465,23 +465,9 @@ __global__ void __launch_bounds__(BLOCK) k_lms32(const u32 *sbits, u32 *lmsbits,
     if (threadIdx.x == 0 && tot) atomicAdd(m1, tot);
 }
 
-// Positions of set bits (ascending) -> out[0..count); if R is non-null,
-// R[p/2] = rank of p (LMS positions are >= 2 apart, so p/2 is unique).
-inline void bits_to_positions(Dev &d, const u32 *bits, u64 nwords, u32 *out, u32 *R) {
-    scan_exclusive(
-        d, nwords, [=] __device__(u64 w) { return (u32)__popc(bits[w]); },
-        [=] __device__(u64 w, u32 off, u32 v) {
-            u32 x = bits[w];
-            while (x) {
-                int b = __ffs(x) - 1;
-                x &= x - 1;
-                u32 p = (u32)(w * 32 + b);
-                out[off] = p;
-                if (R) R[p >> 1] = off;
-                off++;
-            }
-        },
-        nullptr);
+// Positions of set bits (ascending) -> out[0..count).
+inline void bits_to_positions(Dev &d, const u32 *bits, u64 nwords, u32 *out) {
+    bit_list(d, nwords * 32, [=] __device__(u64 w) { return bits[w]; }, out, nullptr);
 }
 
 }  // namespace psk
