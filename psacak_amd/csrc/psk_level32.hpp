@@ -523,27 +523,39 @@ inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
     const u64 cap = np / 8 + 1024;  // anchors are ~1/16 of the positions
     u64 *hk = d.a->get<u64>(cap), *hk2 = d.a->get<u64>(cap);
     u32 *hv = d.a->get<u32>(cap), *hv2 = d.a->get<u32>(cap);
-    // anchors compacted by a scan (32-bit multiplicative hashing, native
-    // multiplies; no shared counter)
+    // anchor bits (one ballot per 64 positions), then the anchors as a list
+    // (at most cap of them) and their window hashes in a dense pass
+    // (32-bit multiplicative hashing, native multiplies; no shared counter)
+    const u64 nwp = (np + 31) / 32;
+    u32 *af = d.a->get<u32>(nwp + 2);
+    for_each_waves(d, np, [=] __device__(u64 i, bool live) {
+        const bool a = live && (rep_hash8(rank, i) >> 28) == 0;
+        const u64 b = ballot(a);
+        if ((lane_id() & 31) == 0 && (i >> 5) < nwp) af[i >> 5] = (u32)(b >> (lane_id() & 32));
+    });
+    u32 *alist = d.a->get<u32>(cap);
     const u64 cc = cap;
-    scan_exclusive(
-        d, np, [=] __device__(u64 i) { return (rep_hash8(rank, i) >> 28) == 0 ? 1u : 0u; },
-        [=] __device__(u64 i, u32 slot, u32 v) {
-            if (!v || slot >= cc) return;
-            u32 h1 = 0x85A308D3u, h2 = 0x13198A2Eu;
-            #pragma unroll 4
-            for (int j = 0; j < REP_WIN; j++) {
-                const u32 r = rank[i + j];
-                h1 = (h1 ^ r) * 0x85EBCA6Bu;
-                h2 = (h2 + r) * 0xC2B2AE35u;
-                h2 ^= h2 >> 13;
-            }
-            hk[slot] = ((u64)h1 << 32) | h2;
-            hv[slot] = 0;
+    bit_list_f(
+        d, np, [=] __device__(u64 w) { return af[w]; },
+        [=] __device__(u64 j, u32 p) {
+            if (j < cc) alist[j] = p;
         },
         ctr);
     u32 na = d.read_u32(ctr);
     if (na > cap) na = (u32)cap;  // (a skewed text; the estimate uses the first cap)
+    for_each(d, na, [=] __device__(u64 j) {
+        const u64 i = alist[j];
+        u32 h1 = 0x85A308D3u, h2 = 0x13198A2Eu;
+        #pragma unroll 4
+        for (int q = 0; q < REP_WIN; q++) {
+            const u32 r = rank[i + q];
+            h1 = (h1 ^ r) * 0x85EBCA6Bu;
+            h2 = (h2 + r) * 0xC2B2AE35u;
+            h2 ^= h2 >> 13;
+        }
+        hk[j] = ((u64)h1 << 32) | h2;
+        hv[j] = 0;
+    });
     double share = 0.0;
     if (na > 1) {
         bool sw = radix_sort_kv(d, na, hk, hv, hk2, hv2, 64);
@@ -671,18 +683,16 @@ inline void final_induce32(Runtime &rt, const u32 *s, u64 m, const u32 *sbits, c
     }
     // every position is an L or an S item: the two sorts write the whole SA
     if (shortL) {
-        const u64 mm = m;
-        const u32 *Li = Litems;
-        scan_exclusive(
-            d, nL,
-            [=] __device__(u64 i) {
-                const u64 q = (u64)Li[i] + 1;
-                return (q >= mm || !getbit(sbits, q)) ? 1u : 0u;
+        // the L positions whose successor is L-type or the sentinel, from the
+        // type bits (positions from m on read as L-type)
+        const u64 nw = (m + 31) / 32;
+        bit_list(
+            d, m,
+            [=] __device__(u64 w) {
+                const u32 S = sbits[w], Sn = w + 1 < nw ? sbits[w + 1] : 0u;
+                return ~S & ~((S >> 1) | (Sn << 31));
             },
-            [=] __device__(u64 i, u32 ex, u32 v) {
-                if (v) ordL[m1 + ex] = Li[i];
-            },
-            nullptr);
+            ordL + m1, nullptr);
         chain_sort(rt, m, RK, ordL, nL, false, rank, tails, k, false, SA, sbits, 1, kpre, m1);
     } else {
         chain_sort(rt, m, RK, Litems, nL, false, rank, tails, k, false, SA, sbits);

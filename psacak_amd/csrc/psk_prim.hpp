@@ -343,9 +343,9 @@ inline void scan_exclusive(Dev &d, u64 n, In in, Out out, u32 *d_total) {
 // store: lane j finds the word holding output j by a binary search over the
 // wave's offsets and the bit by a binary select (a thread writing its own
 // word's positions would store at a stride of the word's count).
-template <bool FUSED, class WF>
+template <bool FUSED, class WF, class Out>
 __global__ void __launch_bounds__(BLOCK) k_bits_emit(WF wordfn, u64 nbits, u64 nwords,
-                                                     const u32 *ta, u32 *out, u32 *total) {
+                                                     const u32 *ta, Out out, u32 *total) {
     __shared__ u32 lds[NWAVES + 1];
     const u64 base = (u64)blockIdx.x * SCAN_TILE;
     u32 run;
@@ -399,14 +399,15 @@ __global__ void __launch_bounds__(BLOCK) k_bits_emit(WF wordfn, u64 nbits, u64 n
                     pos += (u32)width;
                 }
             }
-            if (j < wtot) out[woff + j] = (u32)((wb + (u64)src) * 32 + pos);
+            if (j < wtot) out((u64)woff + j, (u32)((wb + (u64)src) * 32 + pos));
         }
     }
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = run;
 }
 
-template <class WF>
-inline void bit_list(Dev &d, u64 nbits, WF wordfn, u32 *out, u32 *d_total) {
+// out(j, p): the j-th position p (for lists with a capacity of their own).
+template <class WF, class Out>
+inline void bit_list_f(Dev &d, u64 nbits, WF wordfn, Out out, u32 *d_total) {
     const u64 nwords = (nbits + 31) / 32;
     if (nwords == 0) {
         if (d_total) d.zero(d_total, sizeof(u32));
@@ -434,6 +435,11 @@ inline void bit_list(Dev &d, u64 nbits, WF wordfn, u32 *out, u32 *d_total) {
     }
     PSK_LAUNCH_CHECK();
     mk.reset();
+}
+
+template <class WF>
+inline void bit_list(Dev &d, u64 nbits, WF wordfn, u32 *out, u32 *d_total) {
+    bit_list_f(d, nbits, wordfn, [=] __device__(u64 j, u32 p) { out[j] = p; }, d_total);
 }
 
 // Exclusive sum scan for a compaction: out is called only for the items of
