@@ -321,20 +321,44 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
     return true;
 }
 
-// L and S labels for a scan of a bucket-pointer text s[0..m).
-// mode 0: L-scan labels for L positions (bucket head + 1), RK[m] = 0.
-inline void labels_L(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
-    for_each(d, m + 1, [=] __device__(u64 x) {
-        if (x == m) RK[m] = 0;
-        else if (!getbit(sbits, x)) RK[x] = s[x] + 1;
+// Initial labels for the scans of a bucket-pointer text s[0..m): an L
+// position's L-scan label is its bucket head + 1 and an S position's S-scan
+// label the last slot of its bucket's S part + 1 (tail - 1 is the S-type
+// symbol itself; chain_sort is told to convert them): s[x] + 1 either way.
+// labels_L writes s[x] + 1 at every position (the LMS labels of the L-scan
+// then go over the LMS positions) and RK[m] = 0: a plain stream, four
+// positions per thread with 16-byte accesses.
+inline void labels_L(Dev &d, const u32 *s, u64 m, const u32 *, u32 *RK) {
+    const bool vec = ((((uintptr_t)s) | ((uintptr_t)RK)) & 15u) == 0;
+    const u64 nq = vec ? m / 4 : 0;
+    for_each(d, nq + 1, [=] __device__(u64 q) {
+        if (q < nq) {
+            const uint4 v = *(const uint4 *)(s + 4 * q);
+            *(uint4 *)(RK + 4 * q) = make_uint4(v.x + 1, v.y + 1, v.z + 1, v.w + 1);
+        } else {
+            for (u64 x = 4 * nq; x < m; x++) RK[x] = s[x] + 1;
+            RK[m] = 0;
+        }
     });
 }
 
-// S-scan labels: last slot of the bucket's S part + 1 = s[x] + 1 (tail-1 is
-// the S-type symbol itself); chain_sort is told to convert them.
+// The S-scan labels, after labels_L and the L-scan: the L sort writes L
+// positions only and the LMS labels went over the LMS positions, so only
+// those get s[x] + 1 back (four positions per thread, from the type bits).
 inline void labels_S(Dev &d, const u32 *s, u64 m, const u32 *sbits, u32 *RK) {
-    for_each(d, m, [=] __device__(u64 x) {
-        if (getbit(sbits, x)) RK[x] = s[x] + 1;
+    const u64 nq = (m + 3) / 4;
+    for_each(d, nq, [=] __device__(u64 q) {
+        const u64 x = 4 * q;
+        const u32 w = (u32)(x >> 5), b = (u32)(x & 31);
+        const u32 S = sbits[w] >> b;
+        const u32 prev = b ? sbits[w] >> (b - 1) : (w ? sbits[w - 1] >> 31 : 1u);
+        u32 lms = S & ~((S << 1) | (prev & 1u)) & 15u;  // S-type after an L-type position
+        if (m - x < 4) lms &= (1u << (m - x)) - 1u;
+        while (lms) {
+            const u32 j = (u32)__ffs(lms) - 1;
+            lms &= lms - 1;
+            RK[x + j] = s[x + j] + 1;
+        }
     });
 }
 
