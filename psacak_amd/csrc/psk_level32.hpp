@@ -124,8 +124,9 @@ inline void positions_of(Dev &d, u64 n, WF wordfn, u32 *out) {
     bit_list(d, n, wordfn, out, nullptr);
 }
 
-// Sort items[0..A) by chain keys (see above); RK holds the labels of every
-// position 0..m (m = sentinel) and receives the final labels of the items.
+// Sort items[0..A) (null: the items 0..A-1) by chain keys (see above); RK
+// holds the labels of every position 0..m (m = sentinel) and receives the
+// final labels of the items.
 // If last_slot_labels, the items' initial group labels name the group's last
 // slot + 1 instead of its first (S parts are known by their tail); the first
 // round converts them, so later rounds use first-slot labels.
@@ -217,7 +218,7 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
             // round 1 reads the caller's items and makes the working copy
             u32 p;
             if (first) {
-                p = items[i];
+                p = items ? items[i] : (u32)i;
                 acw[i] = p;
             } else {
                 p = ac[i];
@@ -614,7 +615,6 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const
     Dev &d = rt.d;
     ArenaScope mk(*d.a);
     u32 *RK = d.a->get<u32>(m + 1);
-    u32 *items = d.a->get<u32>(m);
     const u64 mm = m;
     for_each(d, m + 1, [=] __device__(u64 x) {
         if (x == mm) {
@@ -622,10 +622,10 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const
         } else {
             const u32 c = rank[x];
             RK[x] = (c ? tails[c - 1] : 0u) + 1;
-            items[x] = (u32)x;
         }
     });
-    if (!chain_sort(rt, m, RK, items, (u32)m, false, rank, tails, k, true, SA)) {
+    // the items are every position in order (no list: round 1 takes p = i)
+    if (!chain_sort(rt, m, RK, nullptr, (u32)m, false, rank, tails, k, true, SA)) {
         mk.reset();
         return false;
     }
