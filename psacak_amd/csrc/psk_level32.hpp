@@ -330,14 +330,18 @@ inline bool chain_sort(Runtime &rt, u64 m, u32 *RK, const u32 *items, u32 A,
 // then go over the LMS positions) and RK[m] = 0: a plain stream, four
 // positions per thread with 16-byte accesses.
 inline void labels_L(Dev &d, const u32 *s, u64 m, const u32 *, u32 *RK) {
-    const bool vec = ((((uintptr_t)s) | ((uintptr_t)RK)) & 15u) == 0;
-    const u64 nq = vec ? m / 4 : 0;
+    if (((((uintptr_t)s) | ((uintptr_t)RK)) & 15u) != 0) {
+        // (a caller's text off 16-byte alignment) one position per thread
+        for_each(d, m + 1, [=] __device__(u64 x) { RK[x] = x < m ? s[x] + 1 : 0u; });
+        return;
+    }
+    const u64 nq = m / 4;
     for_each(d, nq + 1, [=] __device__(u64 q) {
         if (q < nq) {
             const uint4 v = *(const uint4 *)(s + 4 * q);
             *(uint4 *)(RK + 4 * q) = make_uint4(v.x + 1, v.y + 1, v.z + 1, v.w + 1);
         } else {
-            for (u64 x = 4 * nq; x < m; x++) RK[x] = s[x] + 1;
+            for (u64 x = 4 * nq; x < m; x++) RK[x] = s[x] + 1;  // at most 3
             RK[m] = 0;
         }
     });

@@ -250,6 +250,27 @@ def test_psacak32_random_bucket_pointer_texts(ctx):
         assert np.array_equal(P.psacak32_raw(u, ctx), oracle.sacak32(tr))
 
 
+def test_psacak32_dev_unaligned_texts(ctx):
+    # u32 device texts 4-byte but not 16-byte aligned (caller buffers at an
+    # element offset) take the scalar typing and label paths
+    import torch
+    z = gen.zipf32(200003, sigma=1 << 12, s=1.1, repeat_frac=0.2, seed=12)
+    tr = oracle.translate32(z)
+    assert tr is not None
+    n = len(z)
+    for name, text, fn, exp in (("raw", z, P.psacak32_raw_dev, oracle.sacak32_raw(z)),
+                                ("bucket-pointer", tr, P.psacak32_dev, oracle.sacak32(tr))):
+        base = np.zeros(n + 4, dtype=np.uint32)
+        for off in (0, 1, 2, 3):
+            note(f"psacak32 {name} device text at element offset {off}")
+            base[off:off + n] = text
+            dt_all = torch.from_numpy(base.view(np.int32)).to("cuda:0")
+            ds = torch.empty(n, dtype=torch.int32, device="cuda:0")
+            fn(dt_all[off:off + n], ds, ctx=ctx)
+            torch.cuda.synchronize()
+            assert np.array_equal(ds.cpu().numpy().view(np.uint32), exp), (name, off)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_psacak32_long_equal_runs(ctx, seed):
     # long runs of one symbol keep the items of a type run tied for many
