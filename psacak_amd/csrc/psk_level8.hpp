@@ -437,6 +437,35 @@ inline void psacak32_run(Runtime &rt, const u32 *s, u64 m, u32 *SA) {
     mk.reset();
 }
 
+// Run boundaries of sorted symbol keys K[0..n) (values < n): hd[x] / tl[x]
+// = first / last sorted index of symbol x.  The run starts are summed per
+// workgroup and added to one of RUN_COUNTERS counters (by workgroup), so the
+// count takes a few thousand atomics, not one per wave that meets a run
+// start (those serialise on a shared address).
+constexpr int RUN_COUNTERS = 64;
+__global__ void __launch_bounds__(BLOCK) k_run_ends(const u64 *K, u64 n, u32 *hd, u32 *tl,
+                                                    u32 *cs) {
+    __shared__ u32 wc[NWAVES];
+    u32 c = 0;
+    for (u64 i = (u64)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (u64)gridDim.x * BLOCK) {
+        const u64 x = K[i];
+        if (i == 0 || K[i - 1] != x) {
+            hd[x] = (u32)i;
+            c++;
+        }
+        if (i + 1 == n || K[i + 1] != x) tl[x] = (u32)i;
+    }
+    #pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+    if (lane_id() == 0) wc[wave_id()] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 tot = 0;
+        for (int w = 0; w < NWAVES; w++) tot += wc[w];
+        if (tot) atomicAdd(cs + (blockIdx.x % RUN_COUNTERS), tot);
+    }
+}
+
 // translate_sample (psacak32.rs:912-943) on the device: rank the symbols,
 // L -> head of the symbol's bucket, S -> tail - 1.  Returns false (and writes
 // the SA directly) if every symbol is distinct.
@@ -452,15 +481,45 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
     // keys, and the largest symbol (the sort needs only its bits); a wave
     // raises the maximum only when it beats the value already stored, so the
     // same-address atomics stay few
-    for_each_waves(d, n, [=] __device__(u64 i, bool live) {
-        const u32 x = live ? t[i] : 0u;
-        if (live) {
-            k[i] = x;
-            v[i] = (u32)i;
-        }
-        const u32 wm = wave_max(x);
-        if (lane_id() == 0 && wm > __atomic_load_n(mx, __ATOMIC_RELAXED)) atomicMax(mx, wm);
-    });
+    if ((((uintptr_t)t) & 15u) == 0) {
+        // four symbols per lane with 16-byte accesses
+        const u64 nq = (n + 3) / 4, nn = n;
+        for_each_waves(d, nq, [=] __device__(u64 q, bool live) {
+            u32 x4[4] = {0u, 0u, 0u, 0u};
+            const u64 i = 4 * q;
+            if (live) {
+                if (i + 4 <= nn) {
+                    const uint4 x = *(const uint4 *)(t + i);
+                    x4[0] = x.x;
+                    x4[1] = x.y;
+                    x4[2] = x.z;
+                    x4[3] = x.w;
+                    *(ulonglong2 *)(k + i) = make_ulonglong2(x.x, x.y);
+                    *(ulonglong2 *)(k + i + 2) = make_ulonglong2(x.z, x.w);
+                    *(uint4 *)(v + i) = make_uint4((u32)i, (u32)i + 1, (u32)i + 2, (u32)i + 3);
+                } else {
+                    for (u64 j = i; j < nn; j++) {
+                        x4[j - i] = t[j];
+                        k[j] = t[j];
+                        v[j] = (u32)j;
+                    }
+                }
+            }
+            const u32 m01 = x4[0] > x4[1] ? x4[0] : x4[1], m23 = x4[2] > x4[3] ? x4[2] : x4[3];
+            const u32 wm = wave_max(m01 > m23 ? m01 : m23);
+            if (lane_id() == 0 && wm > __atomic_load_n(mx, __ATOMIC_RELAXED)) atomicMax(mx, wm);
+        });
+    } else {
+        for_each_waves(d, n, [=] __device__(u64 i, bool live) {
+            const u32 x = live ? t[i] : 0u;
+            if (live) {
+                k[i] = x;
+                v[i] = (u32)i;
+            }
+            const u32 wm = wave_max(x);
+            if (lane_id() == 0 && wm > __atomic_load_n(mx, __ATOMIC_RELAXED)) atomicMax(mx, wm);
+        });
+    }
     const u32 maxv = d.read_u32(mx);
     const int bits = std::max(1, bit_width(maxv));
     bool sw = radix_sort_kv(d, n, k, v, kt, vt, bits);
@@ -473,27 +532,17 @@ inline bool translate32_run(Runtime &rt, const u32 *t, u64 n, u32 *out, u32 *SA)
         // sorted order), then every position looks its value up in text order,
         // instead of a random scatter of the whole text
         u32 *hd = first, *tl = d.a->get<u32>(n);
-        // run starts counted on 64 counters (a wave with a run start adds to
-        // the counter of its 64-position block mod 64): one shared counter
-        // serialised ~10^6 same-address atomics (4.9 ms for 2^28 symbols)
-        constexpr int NC = 64;
-        u32 *cs = d.a->get<u32>(NC);
-        d.zero(cs, NC * sizeof(u32));
-        for_each_waves(d, n, [=] __device__(u64 i, bool live) {
-            bool st = false;
-            if (live) {
-                const u64 x = K[i];
-                st = i == 0 || K[i - 1] != x;
-                if (st) hd[x] = (u32)i;
-                if (i + 1 == nn || K[i + 1] != x) tl[x] = (u32)i;
-            }
-            const u64 b = ballot(st);
-            if (lane_id() == 0 && b) atomicAdd(cs + ((i / WAVE) & (NC - 1)), (u32)__popcll(b));
-        });
-        u32 csum[NC];
+        u32 *cs = d.a->get<u32>(RUN_COUNTERS);
+        d.zero(cs, RUN_COUNTERS * sizeof(u32));
+        {
+            const u32 g = (u32)std::min<u64>(ceil_div(n, BLOCK), 8192);
+            k_run_ends<<<g, BLOCK, 0, d.s>>>(K, n, hd, tl, cs);
+            PSK_LAUNCH_CHECK();
+        }
+        u32 csum[RUN_COUNTERS];
         d.read(csum, cs, sizeof(csum));
         u64 dsum = 0;
-        for (int c = 0; c < NC; c++) dsum += csum[c];
+        for (int c = 0; c < RUN_COUNTERS; c++) dsum += csum[c];
         const u32 distinct = (u32)std::min<u64>(dsum, n);
         if (distinct >= n) {
             PSK_HIP(hipMemcpyAsync(SA, V, n * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
