@@ -406,18 +406,21 @@ inline void batch8_run(Runtime &rt, const u8 *T, const u64 *off, u64 K, u32 *SA)
         return lo;
     };
     for_each(d, nb, [=] __device__(u64 b) { bt[b] = (u32)last_le(0, KK, b << BTS); });
+    // the values (positions inside their text) replace the SA entries in
+    // place and the sort starts from SA, so an even number of passes leaves
+    // the result in SA with no copy
     u64 *k0 = d.a->get<u64>(N), *k1 = d.a->get<u64>(N);
-    u32 *v0 = d.a->get<u32>(N), *v1 = d.a->get<u32>(N);
+    u32 *v1 = d.a->get<u32>(N);
     for_each(d, N, [=] __device__(u64 j) {
         const u32 x = SA[j];
         const u64 b = x >> BTS;
         const u64 hi = b + 1 < nb ? (u64)bt[b + 1] + 1 : KK;
         const u64 i = last_le(bt[b], hi < KK ? hi : KK, x);
         k0[j] = i;
-        v0[j] = x - doff[i];
+        SA[j] = x - doff[i];
     });
-    bool sw = radix_sort_kv(d, N, k0, v0, k1, v1, bit_width(K > 1 ? K - 1 : 0));
-    PSK_HIP(hipMemcpyAsync(SA, sw ? v1 : v0, N * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
+    const bool sw = radix_sort_kv(d, N, k0, SA, k1, v1, bit_width(K > 1 ? K - 1 : 0));
+    if (sw) PSK_HIP(hipMemcpyAsync(SA, v1, N * sizeof(u32), hipMemcpyDeviceToDevice, d.s));
     mk.reset();
 }
 
