@@ -606,7 +606,10 @@ inline double repeat_share(Dev &d, const u32 *rank, u64 m) {
 // SA of a level by prefix doubling over all its suffixes: every position is an
 // item, its initial label is the first slot of its symbol's bucket (+1), the
 // sentinel (label 0) is the only terminal, and chain_sort's rounds compare
-// (label(p), label(p+h)) with h doubling.  A suffix that still ties after a
+// (label(p), label(p+h)) with h doubling.  A position whose symbol occurs
+// once is final from the start (its bucket is its slot): only the others go
+// through the rounds (almost none in the deep levels of texts over large
+// alphabets, whose names are nearly all distinct).  A suffix that still ties after a
 // round has its first 2h symbols in common with another and no sentinel
 // among them, so p + h stays <= m.  When almost every symbol of the level is
 // distinct (the names of a deep level), few suffixes tie for long and this
@@ -619,17 +622,31 @@ inline bool level32_doubling(Runtime &rt, u64 m, u32 *SA, const u32 *rank, const
     Dev &d = rt.d;
     ArenaScope mk(*d.a);
     u32 *RK = d.a->get<u32>(m + 1);
-    const u64 mm = m;
-    for_each(d, m + 1, [=] __device__(u64 x) {
-        if (x == mm) {
-            RK[x] = 0;
-        } else {
-            const u32 c = rank[x];
-            RK[x] = (c ? tails[c - 1] : 0u) + 1;
+    const u64 mm = m, nw = (m + 31) / 32;
+    // initial labels; a position whose symbol occurs once is final already
+    // (its SA entry is its bucket's slot), the others are flagged as items
+    u32 *flag = d.a->get<u32>(nw + 2);
+    u32 *cnt = d.a->get<u32>(1);
+    for_each_waves(d, m + 1, [=] __device__(u64 x, bool live) {
+        bool item = false;
+        if (live) {
+            if (x == mm) {
+                RK[x] = 0;
+            } else {
+                const u32 c = rank[x];
+                const u32 h = c ? tails[c - 1] : 0u;
+                RK[x] = h + 1;
+                if (tails[c] - h == 1) SA[h] = (u32)x;
+                else item = true;
+            }
         }
+        const u64 b = ballot(item);
+        if ((lane_id() & 31) == 0 && (x >> 5) < nw) flag[x >> 5] = (u32)(b >> (lane_id() & 32));
     });
-    // the items are every position in order (no list: round 1 takes p = i)
-    if (!chain_sort(rt, m, RK, nullptr, (u32)m, false, rank, tails, k, true, SA)) {
+    u32 *items = d.a->get<u32>(m);
+    bit_list(d, m, [=] __device__(u64 w) { return flag[w]; }, items, cnt);
+    const u32 A = d.read_u32(cnt);
+    if (!chain_sort(rt, m, RK, items, A, false, rank, tails, k, true, SA)) {
         mk.reset();
         return false;
     }
