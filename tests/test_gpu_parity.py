@@ -237,6 +237,25 @@ def test_doubling_levels(threshold):
         assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
 
 
+@pytest.mark.parametrize("sigma", [64, 200, 256])
+def test_doubling_singleton_symbols(sigma):
+    # random texts over large alphabets: the names of their level-1 text are
+    # nearly all distinct, so prefix doubling places almost every position
+    # directly (a symbol that occurs once) and sorts only the few repeated
+    # ones; a copied block keeps some repeated names
+    rng = np.random.default_rng(sigma)
+    with P.Context() as c:
+        c.set_doubling(1)
+        for n in (1 << 16, 1 << 20, (1 << 20) + 12345):
+            t = rng.integers(0, sigma, n).astype(np.uint8)
+            t[n // 2:n // 2 + 5000] = t[100:5100]
+            note(f"doubling singletons sigma={sigma} n={n}")
+            assert np.array_equal(P.psacak(t.tobytes(), c), oracle.sacak8(t)), n
+        z = gen.zipf32(1 << 20, sigma=1 << 18, s=0.6, repeat_frac=0.01, seed=sigma)
+        note(f"doubling singletons zipf32 sigma=2^18 (seed {sigma})")
+        assert np.array_equal(P.psacak32_raw(z, c), oracle.sacak32_raw(z))
+
+
 def test_psacak32_random_bucket_pointer_texts(ctx):
     rng = np.random.default_rng(99)
     for it in range(150):
